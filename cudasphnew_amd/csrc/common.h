@@ -1,0 +1,197 @@
+// common.h -- device-side types and helpers shared by the SPH kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph.h"
+
+namespace sph {
+
+constexpr uint32_t kEmpty = SPH_EMPTY_CELL;
+constexpr uint32_t kStatusOutOfDomain = 1u;
+constexpr uint32_t kStatusNonFinite = 2u;
+constexpr uint32_t kStatusCapacity = 4u;
+
+// Per-scene constants, one record per scene in device memory.  Replaces the
+// module-global __constant__ SimulationParameters and boundary constants of
+// the reference (particle_simulation.cu:26,42-46) so that any number of
+// scenes (and handles) can coexist.
+struct DevScene {
+    // cell grid (gridOrigin/gridSpacing/gridDim, h:56-58)
+    float ox, oy, oz, cell;
+    int nx, ny, nz;
+    uint32_t cell_offset;      // first global key of this scene
+    // sub-particle cell grid (h:59-60)
+    float cell_sub;
+    int snx, sny, snz;
+    // physics (h:64-98)
+    float h, h2, mass, stiff, rho0, mu, gravity, sigma, norm_thresh, dt;
+    float poly6, grad_poly6, lapl_poly6, grad_spiky, lapl_visc;
+    float h_sub, h2_sub, mass_sub, dt_sub;
+    float poly6_sub, grad_poly6_sub, lapl_poly6_sub, grad_spiky_sub, lapl_visc_sub;
+    // boundary grid (cu:42-53): texel coordinate t = (x - bo) * bscale - 0.5
+    float box_, boy_, boz_;
+    float bscale_x, bscale_y, bscale_z;  // dims / L
+    float gstep_x, gstep_y, gstep_z;     // dims / (dims - 1): texels per spacing
+    float ghalf_x, ghalf_y, ghalf_z;     // (dims - 1) / 2
+    float bspacing, r0;
+    int bnx, bny, bnz;
+    uint32_t pad0;
+    uint64_t grid_offset;                 // first node in the grid pool
+    float restitution;
+    int collision;
+    // two-scale split region
+    float split_lo[3], split_hi[3];
+    int sub_placement;
+    uint32_t p_begin, p_end;              // particle range of the scene
+};
+
+// ---------------------------------------------------------------------------
+// Cell coordinates.  These must produce bit-identical keys to the FP32
+// reference expressions; no fast-math, IEEE division.
+
+// compute_particle_hash per-axis coordinate (particle_simulation.cu:101-115):
+// (int)((x - o) / d) % dim.  A coordinate <= -1 (negative hash in the
+// reference) becomes 0, >= dim wraps like "% dim"; both flag out-of-domain.
+__device__ __forceinline__ int hash_coord(float x, float o, float d, int dim, uint32_t& flags) {
+    float f = (x - o) / d;
+    if (!(f == f)) { flags |= kStatusNonFinite; return 0; }
+    if (f <= -1.0f) { flags |= kStatusOutOfDomain; return 0; }
+    if (f >= 2147483520.0f) { flags |= kStatusOutOfDomain; return 0; }
+    int c = (int)f;
+    if (c >= dim) { flags |= kStatusOutOfDomain; c %= dim; }
+    return c;
+}
+
+// compute_grid_coordinate (particle_simulation.cu:1238-1258):
+// (unsigned)((x + d - o) / s) % dim, clamped; the float->unsigned
+// conversion saturates at 0 for negative and NaN values.
+__device__ __forceinline__ int range_coord(float x, float dd, float o, float s, int dim) {
+    float f = ((x + dd) - o) / s;
+    if (!(f > 0.0f)) return 0;
+    if (f >= 2147483520.0f) return 0;
+    int c = (int)((unsigned)f) % dim;
+    return min(max(c, 0), dim - 1);
+}
+
+__device__ __forceinline__ uint32_t cell_key(const DevScene& S, float x, float y, float z, uint32_t& flags) {
+    int cx = hash_coord(x, S.ox, S.cell, S.nx, flags);
+    int cy = hash_coord(y, S.oy, S.cell, S.ny, flags);
+    int cz = hash_coord(z, S.oz, S.cell, S.nz, flags);
+    return S.cell_offset + (uint32_t)(S.nx * (S.ny * cz + cy) + cx);
+}
+
+__device__ __forceinline__ uint32_t sub_cell_key(const DevScene& S, float x, float y, float z,
+                                                 uint32_t& flags) {
+    int cx = hash_coord(x, S.ox, S.cell_sub, S.snx, flags);
+    int cy = hash_coord(y, S.oy, S.cell_sub, S.sny, flags);
+    int cz = hash_coord(z, S.oz, S.cell_sub, S.snz, flags);
+    return (uint32_t)(S.snx * (S.sny * cz + cy) + cx);
+}
+
+// Scene of a particle slot: scenes occupy fixed contiguous slot ranges in
+// both storage and sorted order (their keys are disjoint, increasing ranges).
+__device__ __forceinline__ int scene_of(const uint32_t* __restrict__ begins, int nscenes, uint32_t s) {
+    int lo = 0, hi = nscenes - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (begins[mid] <= s) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary grid lookup: normalized coordinates, linear filter, clamp
+// addressing (set_up_3d_float_texture, particle_simulation.cu:3003-3007),
+// in software with exact FP32 weights.  tx,ty,tz are texel coordinates
+// (already u*dims - 0.5).
+
+struct TexCoord {
+    int x0, x1, y0, y1, z0, z1;
+    float ax, ay, az;
+};
+
+__device__ __forceinline__ void tex_setup1(float t, int n, int& i0, int& i1, float& a) {
+    t = fminf(fmaxf(t, -1.0f), (float)n);
+    float fl = floorf(t);
+    a = t - fl;
+    int b = (int)fl;
+    i0 = min(max(b, 0), n - 1);
+    i1 = min(max(b + 1, 0), n - 1);
+}
+
+__device__ __forceinline__ TexCoord tex_setup(const DevScene& S, float tx, float ty, float tz) {
+    TexCoord c;
+    tex_setup1(tx, S.bnx, c.x0, c.x1, c.ax);
+    tex_setup1(ty, S.bny, c.y0, c.y1, c.ay);
+    tex_setup1(tz, S.bnz, c.z0, c.z1, c.az);
+    return c;
+}
+
+template <class T>
+__device__ __forceinline__ T lerp3(const T* __restrict__ g, const DevScene& S, const TexCoord& c);
+
+template <>
+__device__ __forceinline__ float lerp3<float>(const float* __restrict__ g, const DevScene& S,
+                                              const TexCoord& c) {
+    size_t sx = 1, sy = (size_t)S.bnx, sz = (size_t)S.bnx * S.bny;
+    const float* b = g + S.grid_offset;
+    float v000 = b[c.x0 * sx + c.y0 * sy + c.z0 * sz], v100 = b[c.x1 * sx + c.y0 * sy + c.z0 * sz];
+    float v010 = b[c.x0 * sx + c.y1 * sy + c.z0 * sz], v110 = b[c.x1 * sx + c.y1 * sy + c.z0 * sz];
+    float v001 = b[c.x0 * sx + c.y0 * sy + c.z1 * sz], v101 = b[c.x1 * sx + c.y0 * sy + c.z1 * sz];
+    float v011 = b[c.x0 * sx + c.y1 * sy + c.z1 * sz], v111 = b[c.x1 * sx + c.y1 * sy + c.z1 * sz];
+    float c00 = v000 * (1.0f - c.ax) + v100 * c.ax;
+    float c10 = v010 * (1.0f - c.ax) + v110 * c.ax;
+    float c01 = v001 * (1.0f - c.ax) + v101 * c.ax;
+    float c11 = v011 * (1.0f - c.ax) + v111 * c.ax;
+    float c0 = c00 * (1.0f - c.ay) + c10 * c.ay;
+    float c1 = c01 * (1.0f - c.ay) + c11 * c.ay;
+    return c0 * (1.0f - c.az) + c1 * c.az;
+}
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    float s = 1.0f - t;
+    return make_float4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
+}
+
+template <>
+__device__ __forceinline__ float4 lerp3<float4>(const float4* __restrict__ g, const DevScene& S,
+                                                const TexCoord& c) {
+    size_t sx = 1, sy = (size_t)S.bnx, sz = (size_t)S.bnx * S.bny;
+    const float4* b = g + S.grid_offset;
+    float4 c00 = lerp4(b[c.x0 * sx + c.y0 * sy + c.z0 * sz], b[c.x1 * sx + c.y0 * sy + c.z0 * sz], c.ax);
+    float4 c10 = lerp4(b[c.x0 * sx + c.y1 * sy + c.z0 * sz], b[c.x1 * sx + c.y1 * sy + c.z0 * sz], c.ax);
+    float4 c01 = lerp4(b[c.x0 * sx + c.y0 * sy + c.z1 * sz], b[c.x1 * sx + c.y0 * sy + c.z1 * sz], c.ax);
+    float4 c11 = lerp4(b[c.x0 * sx + c.y1 * sy + c.z1 * sz], b[c.x1 * sx + c.y1 * sy + c.z1 * sz], c.ax);
+    return lerp4(lerp4(c00, c10, c.ay), lerp4(c01, c11, c.ay), c.az);
+}
+
+__device__ __forceinline__ void texel_coords(const DevScene& S, float x, float y, float z, float& tx,
+                                             float& ty, float& tz) {
+    tx = (x - S.box_) * S.bscale_x - 0.5f;
+    ty = (y - S.boy_) * S.bscale_y - 0.5f;
+    tz = (z - S.boz_) * S.bscale_z - 0.5f;
+}
+
+// Distance at a texel coordinate.
+__device__ __forceinline__ float dist_at(const float* __restrict__ gd, const DevScene& S, float tx, float ty,
+                                         float tz) {
+    return lerp3<float>(gd, S, tex_setup(S, tx, ty, tz));
+}
+
+// Central-difference gradient of the distance grid in normalized texture
+// coordinates (particle_simulation.cu:429-439): the lookups sit
+// +-spacing/L away in u, i.e. +-dims/(dims-1) texels; the quotient by 2*dX
+// is (dims-1)/2 per texel difference.
+__device__ __forceinline__ float3 dist_grad_normalized(const float* __restrict__ gd, const DevScene& S,
+                                                       float tx, float ty, float tz) {
+    float3 g;
+    g.x = (dist_at(gd, S, tx + S.gstep_x, ty, tz) - dist_at(gd, S, tx - S.gstep_x, ty, tz)) * S.ghalf_x;
+    g.y = (dist_at(gd, S, tx, ty + S.gstep_y, tz) - dist_at(gd, S, tx, ty - S.gstep_y, tz)) * S.ghalf_y;
+    g.z = (dist_at(gd, S, tx, ty, tz + S.gstep_z) - dist_at(gd, S, tx, ty, tz - S.gstep_z)) * S.ghalf_z;
+    return g;
+}
+
+}  // namespace sph

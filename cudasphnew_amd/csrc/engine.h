@@ -1,0 +1,111 @@
+// engine.h -- host orchestration of the SPH step for one or many scenes.
+// Replaces ParticleSimulation's Init/Advance/stage wrappers
+// (particle_simulation.cu:1882-2117, 2320-2590).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "radix_sort.h"
+
+namespace sph {
+
+enum Stage {
+    kStageHash = 0,
+    kStageClear,
+    kStageSort,
+    kStageCells,
+    kStageReorder,
+    kStageDensity,
+    kStageForce,
+    kStageTwoScaleState,
+    kStageTwoScaleCollect,
+    kStageTwoScaleSeed,
+    kStageTwoScaleSubSort,
+    kStageTwoScaleSubPhysics,
+    kNumStages
+};
+
+const char* stage_name(int s);
+
+struct TwoScale;  // two-scale pass state (twoscale.hip)
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nscenes = 0;
+    std::vector<sph_params> params;
+    std::vector<DevScene> hscenes;
+    std::vector<uint32_t> hbegins;   // nscenes + 1
+    std::vector<size_t> cells_of;    // per scene
+    uint32_t N = 0;
+    uint32_t total_cells = 0;
+    int key_bits = 0;
+
+    // device memory
+    DevScene* d_scenes = nullptr;
+    uint32_t* d_begins = nullptr;
+    float4* grid4 = nullptr;
+    float* gridd = nullptr;
+    size_t grid_nodes = 0;
+    float4 *pos_a = nullptr, *vel_a = nullptr, *pos_b = nullptr, *vel_b = nullptr, *acc = nullptr;
+    uint32_t *oid_a = nullptr, *oid_b = nullptr, *key = nullptr, *skey = nullptr, *perm = nullptr;
+    uint32_t* slot_of_oid = nullptr;
+    uint8_t *state_a = nullptr, *state_b = nullptr;
+    int2* range = nullptr;
+    uint32_t* status = nullptr;
+    uint32_t* n_dev = nullptr;
+    RadixSort sorter;
+    size_t bytes = 0;
+
+    bool keys_valid = false;
+    bool cells_valid = false;
+    uint32_t steps = 0;
+    std::string err;
+
+    // timing
+    bool timing = false;
+    hipEvent_t ev[kNumStages + 1] = {};
+    float stage_ms[kNumStages] = {};
+    float stage_acc_ms[kNumStages] = {};
+    uint32_t stage_steps = 0;
+    std::vector<int> recorded;   // stages marked during the last enqueue
+
+    // graph capture of the steady-state step
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    TwoScale* ts = nullptr;
+
+    ~Engine();
+    sph_status init(int S, const sph_params* P, const sph_scene* scenes);
+    sph_status step(int n, bool two_scale);
+    sph_status sync();
+    sph_status check_launch(const char* what);
+    StepBuffers buffers() const;
+    void enqueue_step(bool two_scale);
+    void mark(int stage);
+    sph_status fail(sph_status s, const std::string& msg);
+    template <class T>
+    sph_status alloc(T** p, size_t count);
+    sph_status read_field(int scene, sph_field f, void* dst, size_t bytes_out);
+    sph_status upload_params(int scene, const sph_params& p);
+};
+
+DevScene make_dev_scene(const sph_params& P, const sph_boundary& B);
+
+// Two-scale pass (twoscale.hip).
+sph_status twoscale_create(Engine& e);
+void twoscale_destroy(Engine& e);
+void twoscale_enqueue(Engine& e);
+sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
+sph_status twoscale_counts(Engine& e, sph_counts* out);
+size_t twoscale_bytes(const Engine& e);
+
+}  // namespace sph

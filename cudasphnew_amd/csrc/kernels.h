@@ -1,0 +1,44 @@
+// kernels.h -- host-side launchers of the SPH step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace sph {
+
+// Device buffers of one particle set (one or many scenes), cell-sorted.
+struct StepBuffers {
+    uint32_t n = 0;              // particles (all scenes)
+    int nscenes = 0;
+    const DevScene* scenes = nullptr;     // [nscenes]
+    const uint32_t* begins = nullptr;     // [nscenes] first slot of each scene
+    const float4* grid4 = nullptr;        // (distance, density, viscosity, 0) pool
+    const float* gridd = nullptr;         // distance pool
+    // storage order ("a") and cell-sorted order ("b")
+    float4* pos_a = nullptr;
+    float4* vel_a = nullptr;     // (vx, vy, vz, density of last step)
+    uint32_t* oid_a = nullptr;
+    uint8_t* state_a = nullptr;
+    float4* pos_b = nullptr;
+    float4* vel_b = nullptr;     // (vx, vy, vz, density)
+    uint32_t* oid_b = nullptr;
+    uint8_t* state_b = nullptr;
+    float4* acc = nullptr;       // sorted order
+    uint32_t* key = nullptr;     // keys of the storage order
+    uint32_t* skey = nullptr;    // sorted keys
+    uint32_t* perm = nullptr;    // storage slot of each sorted slot (before tie fix-up)
+    uint32_t* slot_of_oid = nullptr;
+    int2* range = nullptr;       // cell -> [start, end) in sorted order
+    uint32_t* status = nullptr;
+};
+
+void launch_hash(hipStream_t st, const StepBuffers& b);
+void launch_cells_clear(hipStream_t st, const StepBuffers& b);
+void launch_cells(hipStream_t st, const StepBuffers& b);
+void launch_reorder(hipStream_t st, const StepBuffers& b);
+void launch_density(hipStream_t st, const StepBuffers& b, int block);
+void launch_force(hipStream_t st, const StepBuffers& b, int block);
+
+}  // namespace sph

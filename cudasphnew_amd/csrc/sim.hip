@@ -1,0 +1,710 @@
+// sim.hip -- host orchestration and the C ABI of the SPH step.
+//
+// One Engine advances one or many scenes.  The reference's stage sequence
+// (ParticleSimulation::Advance, particle_simulation.cu:2085-2117) becomes:
+//   [hash]  clear  sort  cells  reorder  density  force(+integrate)
+// all enqueued on the handle's stream without host synchronisation; counts
+// stay on the device.  Errors cross the ABI as sph_status codes.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "engine.h"
+
+namespace sph {
+
+const char* stage_name(int s) {
+    static const char* names[kNumStages] = {"hash",          "clear_cells",      "sort",
+                                            "cell_ranges",   "reorder",          "density",
+                                            "force",         "ts_state",         "ts_collect",
+                                            "ts_seed_project", "ts_sub_sort",    "ts_sub_physics"};
+    return (s >= 0 && s < kNumStages) ? names[s] : "";
+}
+
+DevScene make_dev_scene(const sph_params& P, const sph_boundary& B) {
+    DevScene S;
+    std::memset(&S, 0, sizeof(S));
+    S.ox = P.grid_origin[0]; S.oy = P.grid_origin[1]; S.oz = P.grid_origin[2];
+    S.cell = P.grid_spacing;
+    S.nx = P.grid_dim[0]; S.ny = P.grid_dim[1]; S.nz = P.grid_dim[2];
+    S.cell_sub = P.grid_spacing_sub;
+    S.snx = P.grid_dim_sub[0]; S.sny = P.grid_dim_sub[1]; S.snz = P.grid_dim_sub[2];
+    S.h = P.compact_support;
+    S.h2 = S.h * S.h;
+    S.mass = P.particle_mass;
+    S.stiff = P.gas_stiffness;
+    S.rho0 = P.rest_density;
+    S.mu = P.dynamic_viscosity;
+    S.gravity = P.gravity;
+    S.sigma = P.tension_coefficient;
+    S.norm_thresh = P.norm_thresh;
+    S.dt = P.time_step;
+    S.poly6 = P.poly6; S.grad_poly6 = P.grad_poly6; S.lapl_poly6 = P.lapl_poly6;
+    S.grad_spiky = P.grad_spiky; S.lapl_visc = P.lapl_visc;
+    S.h_sub = P.compact_support_sub;
+    S.h2_sub = S.h_sub * S.h_sub;
+    S.mass_sub = P.sub_particle_mass;
+    S.dt_sub = P.time_step_sub;
+    S.poly6_sub = P.poly6_sub; S.grad_poly6_sub = P.grad_poly6_sub; S.lapl_poly6_sub = P.lapl_poly6_sub;
+    S.grad_spiky_sub = P.grad_spiky_sub; S.lapl_visc_sub = P.lapl_visc_sub;
+    S.box_ = B.origin[0]; S.boy_ = B.origin[1]; S.boz_ = B.origin[2];
+    int dims[3] = {B.dims[0], B.dims[1], B.dims[2]};
+    float scale[3], gstep[3], ghalf[3];
+    for (int a = 0; a < 3; ++a) {
+        // gBoundaryGridLength (cu:2626-2631) in float, as the reference
+        float L = (float)(dims[a] - 1) * B.spacing;
+        scale[a] = (float)((double)dims[a] / (double)L);
+        gstep[a] = (float)((double)dims[a] / (double)(dims[a] - 1));
+        ghalf[a] = 0.5f * (float)(dims[a] - 1);
+    }
+    S.bscale_x = scale[0]; S.bscale_y = scale[1]; S.bscale_z = scale[2];
+    S.gstep_x = gstep[0]; S.gstep_y = gstep[1]; S.gstep_z = gstep[2];
+    S.ghalf_x = ghalf[0]; S.ghalf_y = ghalf[1]; S.ghalf_z = ghalf[2];
+    S.bspacing = B.spacing;
+    S.r0 = B.rest_distance;
+    S.bnx = dims[0]; S.bny = dims[1]; S.bnz = dims[2];
+    S.restitution = P.restitution;
+    S.collision = P.collision;
+    for (int a = 0; a < 3; ++a) {
+        S.split_lo[a] = P.split_lo[a];
+        S.split_hi[a] = P.split_hi[a];
+    }
+    S.sub_placement = P.sub_placement;
+    return S;
+}
+
+static bool params_valid(const sph_params& P, std::string& why) {
+    for (int a = 0; a < 3; ++a) {
+        if (P.grid_dim[a] < 1) { why = "grid_dim must be >= 1"; return false; }
+        if (!std::isfinite(P.grid_origin[a])) { why = "grid_origin not finite"; return false; }
+    }
+    double cells = (double)P.grid_dim[0] * P.grid_dim[1] * P.grid_dim[2];
+    if (cells > 2147483647.0) { why = "cell grid too large for 32-bit keys"; return false; }
+    if (!(P.grid_spacing > 0.0f)) { why = "grid_spacing must be > 0"; return false; }
+    if (!(P.compact_support > 0.0f)) { why = "compact_support must be > 0"; return false; }
+    if (!(P.particle_mass > 0.0f)) { why = "particle_mass must be > 0"; return false; }
+    if (!(P.rest_density > 0.0f)) { why = "rest_density must be > 0"; return false; }
+    if (!(P.time_step > 0.0f)) { why = "time_step must be > 0"; return false; }
+    return true;
+}
+
+static bool boundary_valid(const sph_boundary& B, std::string& why) {
+    for (int a = 0; a < 3; ++a)
+        if (B.dims[a] < 2) { why = "boundary dims must be >= 2"; return false; }
+    if (!(B.spacing > 0.0f)) { why = "boundary spacing must be > 0"; return false; }
+    if (!B.distance || !B.density || !B.viscosity) { why = "boundary arrays must be non-NULL"; return false; }
+    double nodes = (double)B.dims[0] * B.dims[1] * B.dims[2];
+    if (nodes > 4.0e9) { why = "boundary grid too large"; return false; }
+    return true;
+}
+
+Engine::~Engine() {
+    if (ts) twoscale_destroy(*this);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (int i = 0; i <= kNumStages; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
+                    perm, slot_of_oid, state_a, state_b, range, status, n_dev};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    sorter.release();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+sph_status Engine::fail(sph_status s, const std::string& msg) {
+    err = msg;
+    return s;
+}
+
+template <class T>
+sph_status Engine::alloc(T** p, size_t count) {
+    size_t b = sizeof(T) * (count ? count : 1);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(p), b);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(SPH_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    bytes += b;
+    return SPH_OK;
+}
+
+sph_status Engine::check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(SPH_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+    return SPH_OK;
+}
+
+sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
+    if (S < 1 || !P || !scenes) return fail(SPH_INVALID_ARG, "need at least one scene");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        (void)hipGetLastError();
+        return fail(SPH_NO_DEVICE, "no HIP device visible");
+    }
+    (void)hipGetDevice(&device);
+    nscenes = S;
+    params.assign(P, P + S);
+    hscenes.resize(S);
+    hbegins.assign(S + 1, 0);
+    cells_of.assign(S, 0);
+    uint64_t ncells = 0, np = 0, nodes = 0;
+    for (int s = 0; s < S; ++s) {
+        std::string why;
+        if (!params_valid(P[s], why)) return fail(SPH_INVALID_ARG, "scene " + std::to_string(s) + ": " + why);
+        if (!boundary_valid(scenes[s].boundary, why))
+            return fail(SPH_INVALID_ARG, "scene " + std::to_string(s) + ": " + why);
+        if (scenes[s].num_particles < 1 || !scenes[s].positions)
+            return fail(SPH_INVALID_ARG, "scene " + std::to_string(s) + ": no particles");
+        for (size_t i = 0; i < 3 * (size_t)scenes[s].num_particles; ++i)
+            if (!std::isfinite(scenes[s].positions[i]))
+                return fail(SPH_INVALID_ARG, "scene " + std::to_string(s) + ": non-finite position");
+        params[s].num_particles = scenes[s].num_particles;
+        hscenes[s] = make_dev_scene(params[s], scenes[s].boundary);
+        hscenes[s].cell_offset = (uint32_t)ncells;
+        hscenes[s].grid_offset = nodes;
+        hscenes[s].p_begin = (uint32_t)np;
+        hbegins[s] = (uint32_t)np;
+        cells_of[s] = (size_t)P[s].grid_dim[0] * P[s].grid_dim[1] * P[s].grid_dim[2];
+        ncells += cells_of[s];
+        np += scenes[s].num_particles;
+        hscenes[s].p_end = (uint32_t)np;
+        nodes += (uint64_t)scenes[s].boundary.dims[0] * scenes[s].boundary.dims[1] * scenes[s].boundary.dims[2];
+        if (ncells > 0x7fffffffull) return fail(SPH_INVALID_ARG, "total cells exceed 32-bit keys");
+        if (np > 0x7fffffffull) return fail(SPH_INVALID_ARG, "too many particles");
+    }
+    hbegins[S] = (uint32_t)np;
+    N = (uint32_t)np;
+    total_cells = (uint32_t)ncells;
+    grid_nodes = nodes;
+    key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) < (uint64_t)total_cells) ++key_bits;
+
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "hipStreamCreate failed");
+    sph_status st;
+#define SPH_TRY(x) do { if ((st = (x)) != SPH_OK) return st; } while (0)
+    SPH_TRY(alloc(&d_scenes, S));
+    SPH_TRY(alloc(&d_begins, S + 1));
+    SPH_TRY(alloc(&grid4, nodes));
+    SPH_TRY(alloc(&gridd, nodes));
+    SPH_TRY(alloc(&pos_a, N));
+    SPH_TRY(alloc(&vel_a, N));
+    SPH_TRY(alloc(&pos_b, N));
+    SPH_TRY(alloc(&vel_b, N));
+    SPH_TRY(alloc(&acc, N));
+    SPH_TRY(alloc(&oid_a, N));
+    SPH_TRY(alloc(&oid_b, N));
+    SPH_TRY(alloc(&key, N));
+    SPH_TRY(alloc(&skey, N));
+    SPH_TRY(alloc(&perm, N));
+    SPH_TRY(alloc(&slot_of_oid, N));
+    SPH_TRY(alloc(&state_a, N));
+    SPH_TRY(alloc(&state_b, N));
+    SPH_TRY(alloc(&range, total_cells));
+    SPH_TRY(alloc(&status, 1));
+    SPH_TRY(alloc(&n_dev, 1));
+    if (sorter.init(N) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SPH_OUT_OF_MEMORY, "radix sort buffers");
+    }
+    bytes += sorter.bytes();
+
+    // host staging, then upload
+    std::vector<float4> hp(N), hv(N), g4;
+    std::vector<uint32_t> ho(N);
+    for (int s = 0; s < S; ++s) {
+        const sph_scene& sc = scenes[s];
+        uint32_t b = hbegins[s];
+        for (uint32_t i = 0; i < sc.num_particles; ++i) {
+            const float* p = sc.positions + 3 * (size_t)i;
+            hp[b + i] = make_float4(p[0], p[1], p[2], 0.0f);
+            if (sc.velocities) {
+                const float* v = sc.velocities + 3 * (size_t)i;
+                hv[b + i] = make_float4(v[0], v[1], v[2], 0.0f);
+            } else {
+                hv[b + i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            ho[b + i] = i;   // original id within its scene
+        }
+    }
+    uint64_t off = 0;
+    for (int s = 0; s < S; ++s) {
+        const sph_boundary& B = scenes[s].boundary;
+        size_t nn = (size_t)B.dims[0] * B.dims[1] * B.dims[2];
+        g4.resize(nn);
+        for (size_t q = 0; q < nn; ++q) g4[q] = make_float4(B.distance[q], B.density[q], B.viscosity[q], 0.0f);
+        if (hipMemcpy(grid4 + off, g4.data(), nn * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(gridd + off, B.distance, nn * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(SPH_DEVICE_ERROR, "boundary upload failed");
+        off += nn;
+    }
+    if (hipMemcpy(pos_a, hp.data(), N * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(vel_a, hv.data(), N * sizeof(float4), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(oid_a, ho.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_scenes, hscenes.data(), S * sizeof(DevScene), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_begins, hbegins.data(), (S + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(n_dev, &N, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "state upload failed");
+    // cell tables start empty (Init, cu:1944-1946); state bytes start at 0
+    if (hipMemset(range, 0xFF, (size_t)total_cells * sizeof(int2)) != hipSuccess ||
+        hipMemset(status, 0, sizeof(uint32_t)) != hipSuccess || hipMemset(state_a, 0, N) != hipSuccess ||
+        hipMemset(state_b, 0, N) != hipSuccess || hipMemset(acc, 0, N * sizeof(float4)) != hipSuccess ||
+        hipMemset(pos_b, 0, N * sizeof(float4)) != hipSuccess ||
+        hipMemset(vel_b, 0, N * sizeof(float4)) != hipSuccess ||
+        hipMemset(skey, 0, N * sizeof(uint32_t)) != hipSuccess ||
+        hipMemset(oid_b, 0, N * sizeof(uint32_t)) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "initial clear failed");
+    for (int i = 0; i <= kNumStages; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) return fail(SPH_DEVICE_ERROR, "hipEventCreate failed");
+    if (hipDeviceSynchronize() != hipSuccess) return fail(SPH_DEVICE_ERROR, "device sync after create failed");
+#undef SPH_TRY
+    return SPH_OK;
+}
+
+StepBuffers Engine::buffers() const {
+    StepBuffers b;
+    b.n = N;
+    b.nscenes = nscenes;
+    b.scenes = d_scenes;
+    b.begins = d_begins;
+    b.grid4 = grid4;
+    b.gridd = gridd;
+    b.pos_a = pos_a; b.vel_a = vel_a; b.oid_a = oid_a; b.state_a = ts ? state_a : nullptr;
+    b.pos_b = pos_b; b.vel_b = vel_b; b.oid_b = oid_b; b.state_b = ts ? state_b : nullptr;
+    b.acc = acc;
+    b.key = key;
+    b.skey = skey;
+    b.perm = perm;
+    b.slot_of_oid = slot_of_oid;
+    b.range = range;
+    b.status = status;
+    return b;
+}
+
+void Engine::mark(int stage) {
+    if (!timing) return;
+    (void)hipEventRecord(ev[stage + 1], stream);
+    recorded.push_back(stage);
+}
+
+void Engine::enqueue_step(bool two_scale) {
+    StepBuffers b = buffers();
+    recorded.clear();
+    if (timing) (void)hipEventRecord(ev[0], stream);
+    if (!keys_valid) launch_hash(stream, b);
+    mark(kStageHash);
+    if (cells_valid) launch_cells_clear(stream, b);
+    mark(kStageClear);
+    sorter.sort(stream, key, nullptr, skey, perm, n_dev, key_bits);
+    mark(kStageSort);
+    launch_cells(stream, b);
+    mark(kStageCells);
+    launch_reorder(stream, b);
+    mark(kStageReorder);
+    launch_density(stream, b, 256);
+    mark(kStageDensity);
+    if (two_scale) twoscale_enqueue(*this);
+    launch_force(stream, b, 256);
+    mark(kStageForce);
+    keys_valid = true;
+    cells_valid = true;
+}
+
+sph_status Engine::step(int n, bool two_scale) {
+    if (n < 0) return fail(SPH_INVALID_ARG, "n_steps must be >= 0");
+    if (two_scale && !ts) {
+        sph_status st = twoscale_create(*this);
+        if (st != SPH_OK) return st;
+    }
+    for (int i = 0; i < n; ++i) {
+        bool graphable = use_graph && !timing && keys_valid && cells_valid && !two_scale;
+        if (graphable) {
+            if (!graph_exec) {
+                if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                    return fail(SPH_DEVICE_ERROR, "hipStreamBeginCapture failed");
+                enqueue_step(false);
+                if (hipStreamEndCapture(stream, &graph) != hipSuccess)
+                    return fail(SPH_DEVICE_ERROR, "hipStreamEndCapture failed");
+                if (hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                    return fail(SPH_DEVICE_ERROR, "hipGraphInstantiate failed");
+            }
+            if (hipGraphLaunch(graph_exec, stream) != hipSuccess)
+                return fail(SPH_DEVICE_ERROR, "hipGraphLaunch failed");
+        } else {
+            enqueue_step(two_scale);
+            if (timing) {
+                // stage durations: consecutive events in enqueue order
+                hipEvent_t last = ev[recorded.back() + 1];
+                if (hipEventSynchronize(last) != hipSuccess) return fail(SPH_DEVICE_ERROR, "timing sync failed");
+                hipEvent_t prev = ev[0];
+                for (int s : recorded) {
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, prev, ev[s + 1]);
+                    stage_ms[s] = ms;
+                    stage_acc_ms[s] += ms;
+                    prev = ev[s + 1];
+                }
+                ++stage_steps;
+            }
+        }
+        sph_status st = check_launch("step");
+        if (st != SPH_OK) return st;
+        ++steps;
+    }
+    return SPH_OK;
+}
+
+sph_status Engine::sync() {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(SPH_DEVICE_ERROR, std::string("stream sync: ") + hipGetErrorString(e));
+    uint32_t s = 0;
+    if (hipMemcpy(&s, status, sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "status read failed");
+    if (s & kStatusNonFinite) return fail(SPH_NONFINITE_STATE, "non-finite particle state");
+    if (s & kStatusCapacity) return fail(SPH_CAPACITY_EXCEEDED, "sub-particle capacity exceeded");
+    if (s & kStatusOutOfDomain) return fail(SPH_OUT_OF_DOMAIN, "particle left the cell grid");
+    return SPH_OK;
+}
+
+sph_status Engine::upload_params(int s, const sph_params& p) {
+    const sph_params& o = params[s];
+    for (int a = 0; a < 3; ++a)
+        if (o.grid_dim[a] != p.grid_dim[a] || o.grid_origin[a] != p.grid_origin[a] ||
+            o.grid_dim_sub[a] != p.grid_dim_sub[a])
+            return fail(SPH_INVALID_ARG, "grid geometry may not change");
+    if (o.grid_spacing != p.grid_spacing || o.grid_spacing_sub != p.grid_spacing_sub)
+        return fail(SPH_INVALID_ARG, "grid spacing may not change");
+    std::string why;
+    if (!params_valid(p, why)) return fail(SPH_INVALID_ARG, why);
+    DevScene d = hscenes[s];
+    // keep boundary and layout fields, refresh the physics
+    sph_boundary B;
+    std::memset(&B, 0, sizeof(B));
+    B.origin[0] = d.box_; B.origin[1] = d.boy_; B.origin[2] = d.boz_;
+    B.spacing = d.bspacing;
+    B.dims[0] = d.bnx; B.dims[1] = d.bny; B.dims[2] = d.bnz;
+    B.rest_distance = d.r0;
+    DevScene n = make_dev_scene(p, B);
+    n.cell_offset = d.cell_offset;
+    n.grid_offset = d.grid_offset;
+    n.p_begin = d.p_begin;
+    n.p_end = d.p_end;
+    hscenes[s] = n;
+    params[s] = p;
+    params[s].num_particles = d.p_end - d.p_begin;
+    if (hipMemcpyAsync(d_scenes + s, &hscenes[s], sizeof(DevScene), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "parameter upload failed");
+    if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "parameter upload sync");
+    if (graph_exec) {
+        (void)hipGraphExecDestroy(graph_exec);
+        (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+    return SPH_OK;
+}
+
+sph_status Engine::read_field(int s, sph_field f, void* dst, size_t nbytes) {
+    if (s < 0 || s >= nscenes) return fail(SPH_INVALID_ARG, "scene index out of range");
+    if (!dst && nbytes) return fail(SPH_INVALID_ARG, "NULL destination");
+    sph_status st = sync();
+    if (st != SPH_OK && st != SPH_OUT_OF_DOMAIN && st != SPH_NONFINITE_STATE) return st;
+    const uint32_t b = hbegins[s], n = hbegins[s + 1] - hbegins[s];
+    auto need = [&](size_t want) -> sph_status {
+        if (nbytes != want) {
+            char buf[128];
+            std::snprintf(buf, sizeof(buf), "field %d needs %zu bytes, got %zu", (int)f, want, nbytes);
+            return fail(SPH_INVALID_ARG, buf);
+        }
+        return SPH_OK;
+    };
+    auto copy = [&](void* host, const void* dev, size_t bytes_) -> sph_status {
+        if (hipMemcpy(host, dev, bytes_, hipMemcpyDeviceToHost) != hipSuccess)
+            return fail(SPH_DEVICE_ERROR, "device-to-host copy failed");
+        return SPH_OK;
+    };
+    std::vector<uint32_t> oid(n);
+    auto load_oid = [&](const uint32_t* src) { return copy(oid.data(), src + b, n * sizeof(uint32_t)); };
+    switch (f) {
+    case SPH_FIELD_POSITION:
+    case SPH_FIELD_VELOCITY:
+    case SPH_FIELD_ACCEL: {
+        if ((st = need(3 * (size_t)n * sizeof(float))) != SPH_OK) return st;
+        const float4* src = f == SPH_FIELD_POSITION ? pos_a : (f == SPH_FIELD_VELOCITY ? vel_a : acc);
+        std::vector<float4> tmp(n);
+        if ((st = copy(tmp.data(), src + b, n * sizeof(float4))) != SPH_OK) return st;
+        if ((st = load_oid(oid_a)) != SPH_OK) return st;
+        float* o = (float*)dst;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t id = oid[i];
+            o[3 * (size_t)id] = tmp[i].x;
+            o[3 * (size_t)id + 1] = tmp[i].y;
+            o[3 * (size_t)id + 2] = tmp[i].z;
+        }
+        return SPH_OK;
+    }
+    case SPH_FIELD_DENSITY:
+    case SPH_FIELD_PRESSURE: {
+        if ((st = need((size_t)n * sizeof(float))) != SPH_OK) return st;
+        std::vector<float4> tmp(n);
+        if ((st = copy(tmp.data(), vel_a + b, n * sizeof(float4))) != SPH_OK) return st;
+        if ((st = load_oid(oid_a)) != SPH_OK) return st;
+        float* o = (float*)dst;
+        const sph_params& P = params[s];
+        for (uint32_t i = 0; i < n; ++i) {
+            float rho = tmp[i].w;
+            o[oid[i]] = f == SPH_FIELD_DENSITY ? rho : P.gas_stiffness * (rho - P.rest_density);
+        }
+        return SPH_OK;
+    }
+    case SPH_FIELD_STATE: {
+        if ((st = need(n)) != SPH_OK) return st;
+        std::vector<uint8_t> tmp(n);
+        if ((st = copy(tmp.data(), state_a + b, n)) != SPH_OK) return st;
+        if ((st = load_oid(oid_a)) != SPH_OK) return st;
+        uint8_t* o = (uint8_t*)dst;
+        for (uint32_t i = 0; i < n; ++i) o[oid[i]] = tmp[i];
+        return SPH_OK;
+    }
+    case SPH_FIELD_SORTED_KEYS: {
+        if ((st = need((size_t)n * sizeof(int32_t))) != SPH_OK) return st;
+        if ((st = copy(dst, skey + b, n * sizeof(uint32_t))) != SPH_OK) return st;
+        int32_t* o = (int32_t*)dst;
+        for (uint32_t i = 0; i < n; ++i) o[i] -= (int32_t)hscenes[s].cell_offset;
+        return SPH_OK;
+    }
+    case SPH_FIELD_SORTED_IDS: {
+        if ((st = need((size_t)n * sizeof(int32_t))) != SPH_OK) return st;
+        return copy(dst, oid_b + b, n * sizeof(uint32_t));
+    }
+    case SPH_FIELD_CELL_START:
+    case SPH_FIELD_CELL_END: {
+        size_t nc = cells_of[s];
+        if ((st = need(nc * sizeof(int32_t))) != SPH_OK) return st;
+        std::vector<int2> tmp(nc);
+        if ((st = copy(tmp.data(), range + hscenes[s].cell_offset, nc * sizeof(int2))) != SPH_OK) return st;
+        int32_t* o = (int32_t*)dst;
+        for (size_t c = 0; c < nc; ++c) {
+            int v = f == SPH_FIELD_CELL_START ? tmp[c].x : tmp[c].y;
+            o[c] = v < 0 ? -1 : v - (int)b;
+        }
+        return SPH_OK;
+    }
+    default:
+        if (ts && s == 0) return twoscale_read(*this, f, dst, nbytes);
+        return fail(SPH_INVALID_ARG, "field not available");
+    }
+}
+
+}  // namespace sph
+
+using sph::Engine;
+
+struct sph_handle {
+    Engine e;
+};
+
+struct sph_batch {
+    Engine e;
+};
+
+static thread_local std::string g_create_error;
+
+extern "C" {
+
+int32_t sph_abi_version(void) { return SPH_ABI_VERSION; }
+
+int32_t sph_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+sph_status sph_create(const sph_params* params, const sph_scene* scene, sph_handle** out) {
+    if (!out) return SPH_INVALID_ARG;
+    *out = nullptr;
+    if (!params || !scene) {
+        g_create_error = "NULL params or scene";
+        return SPH_INVALID_ARG;
+    }
+    if (scene->num_particles != params->num_particles) {
+        g_create_error = "scene->num_particles differs from params->num_particles";
+        return SPH_INVALID_ARG;
+    }
+    sph_handle* h = new (std::nothrow) sph_handle();
+    if (!h) return SPH_OUT_OF_MEMORY;
+    sph_status st = h->e.init(1, params, scene);
+    if (st != SPH_OK) {
+        g_create_error = h->e.err;
+        delete h;
+        return st;
+    }
+    g_create_error.clear();
+    *out = h;
+    return SPH_OK;
+}
+
+sph_status sph_destroy(sph_handle* h) {
+    if (!h) return SPH_INVALID_ARG;
+    (void)hipStreamSynchronize(h->e.stream);
+    delete h;
+    return SPH_OK;
+}
+
+sph_status sph_step(sph_handle* h, int32_t n) { return h ? h->e.step(n, false) : SPH_INVALID_ARG; }
+
+sph_status sph_step_two_scale(sph_handle* h, int32_t n) { return h ? h->e.step(n, true) : SPH_INVALID_ARG; }
+
+sph_status sph_sync(sph_handle* h) { return h ? h->e.sync() : SPH_INVALID_ARG; }
+
+sph_status sph_set_params(sph_handle* h, const sph_params* p) {
+    if (!h || !p) return SPH_INVALID_ARG;
+    return h->e.upload_params(0, *p);
+}
+
+sph_status sph_get_params(const sph_handle* h, sph_params* out) {
+    if (!h || !out) return SPH_INVALID_ARG;
+    *out = h->e.params[0];
+    return SPH_OK;
+}
+
+sph_status sph_read_positions(sph_handle* h, float* xyz, size_t n) {
+    if (!h || !xyz) return SPH_INVALID_ARG;
+    if (n != h->e.N) {
+        h->e.err = "n must equal the particle count";
+        return SPH_INVALID_ARG;
+    }
+    return h->e.read_field(0, SPH_FIELD_POSITION, xyz, 3 * n * sizeof(float));
+}
+
+sph_status sph_read_field(sph_handle* h, sph_field f, void* dst, size_t bytes) {
+    return h ? h->e.read_field(0, f, dst, bytes) : SPH_INVALID_ARG;
+}
+
+sph_status sph_map_positions(sph_handle* h, const float** dev_pos4, const uint32_t** dev_oid) {
+    if (!h || !dev_pos4 || !dev_oid) return SPH_INVALID_ARG;
+    // After a step the storage arrays hold the cell-sorted order of that
+    // step; before the first step they hold the initial (id) order.
+    *dev_pos4 = reinterpret_cast<const float*>(h->e.pos_a);
+    *dev_oid = h->e.oid_a;
+    return SPH_OK;
+}
+
+sph_status sph_get_counts(sph_handle* h, sph_counts* out) {
+    if (!h || !out) return SPH_INVALID_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->num_particles = h->e.N;
+    out->num_default = h->e.N;
+    out->num_steps = h->e.steps;
+    if (h->e.ts) return sph::twoscale_counts(h->e, out);
+    return SPH_OK;
+}
+
+size_t sph_num_cells(const sph_handle* h) { return h ? h->e.total_cells : 0; }
+
+size_t sph_device_bytes(const sph_handle* h) {
+    return h ? h->e.bytes + (h->e.ts ? sph::twoscale_bytes(h->e) : 0) : 0;
+}
+
+const char* sph_last_error_message(const sph_handle* h) {
+    return h ? h->e.err.c_str() : g_create_error.c_str();
+}
+
+sph_status sph_enable_timing(sph_handle* h, int32_t enable) {
+    if (!h) return SPH_INVALID_ARG;
+    h->e.timing = enable != 0;
+    std::memset(h->e.stage_acc_ms, 0, sizeof(h->e.stage_acc_ms));
+    h->e.stage_steps = 0;
+    return SPH_OK;
+}
+
+sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count) {
+    if (!h || !ms || !count) return SPH_INVALID_ARG;
+    // average per step since timing was enabled
+    uint32_t n = h->e.stage_steps ? h->e.stage_steps : 1;
+    for (int s = 0; s < sph::kNumStages && s < SPH_MAX_STAGES; ++s) ms[s] = h->e.stage_acc_ms[s] / (float)n;
+    *count = sph::kNumStages < SPH_MAX_STAGES ? sph::kNumStages : SPH_MAX_STAGES;
+    return SPH_OK;
+}
+
+const char* sph_stage_name(int32_t stage) { return sph::stage_name(stage); }
+
+static sph_status time_engine(Engine& e, int32_t n, bool two_scale, float* ms) {
+    if (!ms || n < 1) return SPH_INVALID_ARG;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return SPH_DEVICE_ERROR;
+    (void)hipEventRecord(a, e.stream);
+    sph_status st = e.step(n, two_scale);
+    (void)hipEventRecord(b, e.stream);
+    if (hipEventSynchronize(b) != hipSuccess) st = SPH_DEVICE_ERROR;
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t;
+    if (st != SPH_OK) return st;
+    return e.sync();
+}
+
+sph_status sph_time_steps(sph_handle* h, int32_t n, int32_t two_scale, float* ms) {
+    return h ? time_engine(h->e, n, two_scale != 0, ms) : SPH_INVALID_ARG;
+}
+
+// ---- batched scenes ----
+
+sph_status sph_batch_create(int32_t n_scenes, const sph_params* params, const sph_scene* scenes, sph_batch** out) {
+    if (!out) return SPH_INVALID_ARG;
+    *out = nullptr;
+    if (n_scenes < 1 || !params || !scenes) {
+        g_create_error = "need at least one scene";
+        return SPH_INVALID_ARG;
+    }
+    sph_batch* b = new (std::nothrow) sph_batch();
+    if (!b) return SPH_OUT_OF_MEMORY;
+    sph_status st = b->e.init(n_scenes, params, scenes);
+    if (st != SPH_OK) {
+        g_create_error = b->e.err;
+        delete b;
+        return st;
+    }
+    *out = b;
+    return SPH_OK;
+}
+
+sph_status sph_batch_destroy(sph_batch* b) {
+    if (!b) return SPH_INVALID_ARG;
+    (void)hipStreamSynchronize(b->e.stream);
+    delete b;
+    return SPH_OK;
+}
+
+sph_status sph_batch_step(sph_batch* b, int32_t n) { return b ? b->e.step(n, false) : SPH_INVALID_ARG; }
+
+sph_status sph_batch_sync(sph_batch* b) { return b ? b->e.sync() : SPH_INVALID_ARG; }
+
+sph_status sph_batch_read_field(sph_batch* b, int32_t scene, sph_field f, void* dst, size_t bytes) {
+    return b ? b->e.read_field(scene, f, dst, bytes) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_time_steps(sph_batch* b, int32_t n, float* ms) {
+    return b ? time_engine(b->e, n, false, ms) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_use_graph(sph_batch* b, int32_t enable) {
+    if (!b) return SPH_INVALID_ARG;
+    b->e.use_graph = enable != 0;
+    return SPH_OK;
+}
+
+uint64_t sph_batch_num_particles(const sph_batch* b) { return b ? b->e.N : 0; }
+
+const char* sph_batch_last_error_message(const sph_batch* b) { return b ? b->e.err.c_str() : g_create_error.c_str(); }
+
+}  // extern "C"

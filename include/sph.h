@@ -1,0 +1,371 @@
+/*
+ * sph.h -- C ABI of the MI355X (gfx950) SPH time step.
+ *
+ * This is the drop-in boundary for the SPH step of CUDASPHNEW
+ * (CUDASPHNEW/particle_simulation.h, particle_simulation.cu).  Every entry
+ * point below names the reference interface it replaces.  Plain C types
+ * only: pointers, sizes and POD structs.  No exception crosses this
+ * boundary; every call returns an sph_status and the handle keeps a
+ * message (sph_last_error_message).
+ *
+ * Ownership: a handle owns all device memory it allocates.  Caller arrays
+ * are copied at create time.  Pointers returned by sph_map_* are borrowed
+ * and stay valid until the next step or sph_destroy on that handle.
+ *
+ * Threading: one HIP stream per handle.  Calls on one handle are not
+ * thread-safe; distinct handles may be driven from distinct threads.  A step
+ * performs no host synchronisation (so it can be captured in a hipGraph);
+ * errors detected on the device (non-finite state, particle out of the cell
+ * grid) are reported by the next sph_sync or read call.
+ */
+#ifndef SPH_H
+#define SPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1
+
+/* Empty cell marker of the cell tables (particle_simulation.cu:1661). */
+#define SPH_EMPTY_CELL 0xFFFFFFFFu
+
+/* Error codes.  Replaces the C++ exceptions thrown by CUDA_SAFE_CALL /
+ * THROW_EXCEPTION (util.h:21-52), which Advance swallowed
+ * (particle_simulation.cu:2112-2116). */
+typedef enum sph_status {
+    SPH_OK = 0,
+    SPH_INVALID_ARG = 1,
+    SPH_OUT_OF_MEMORY = 2,
+    SPH_DEVICE_ERROR = 3,
+    SPH_NONFINITE_STATE = 4,
+    SPH_OUT_OF_DOMAIN = 5,
+    SPH_NO_DEVICE = 6,
+    SPH_CAPACITY_EXCEEDED = 7
+} sph_status;
+
+/* Simulation parameters.  The first block mirrors struct SimulationParameters
+ * (particle_simulation.h:54-111) field by field; the second block holds the
+ * options this build adds, each with a value that reproduces the reference. */
+typedef struct sph_params {
+    /* --- particle_simulation.h:56-62: uniform cell grids --- */
+    float grid_origin[3];
+    float grid_spacing;
+    int32_t grid_dim[3];
+    float grid_spacing_sub;
+    int32_t grid_dim_sub[3];
+    /* --- particle_simulation.h:64-69 --- */
+    uint32_t num_particles;
+    float compact_support;          /* h */
+    float compact_support_sub;      /* h_sub */
+    int32_t kernel_particles;
+    /* --- particle_simulation.h:72-74 --- */
+    float time_step;
+    float time_step_sub;
+    /* --- particle_simulation.h:76-85 --- */
+    float particle_mass;
+    float sub_particle_mass;
+    float gas_stiffness;
+    float rest_density;
+    float dynamic_viscosity;
+    float gravity;                  /* a_y -= gravity (particle_simulation.cu:471) */
+    float tension_coefficient;
+    float norm_thresh;
+    float fluid_volume;
+    /* --- particle_simulation.h:88-98: kernel coefficients --- */
+    float poly6, grad_poly6, lapl_poly6, grad_spiky, lapl_visc;
+    float poly6_sub, grad_poly6_sub, lapl_poly6_sub, grad_spiky_sub,
+          lapl_visc_sub;
+    /* --- particle_simulation.h:102-110 --- */
+    float cm_distance_thresh;
+    float n_part_thresh;
+    float box_dim[3];
+    float box_cen[3];
+    float restitution;
+
+    /* --- options added by this build --- */
+    /* 0: no collision response (reference Advance, cu:2085-2117);
+     * 1: collision response of collision_handling (cu:742-834) driven by the
+     *    boundary distance grid, applied inside integration. */
+    int32_t collision;
+    /* Two-scale split region, inclusive AABB.  The reference hard-codes
+     * x in [0.2, 0.5] (cu:947): split_lo = {0.2,-inf,-inf},
+     * split_hi = {0.5,+inf,+inf}. */
+    float split_lo[3];
+    float split_hi[3];
+    /* 0: reference seeding, 8 children at radius (3m/(4 pi rho))^(1/3)
+     *    along the cube diagonals (cu:1023-1090);
+     * 1: children at (+-h/4, +-h/4, +-h/4) around the parent. */
+    int32_t sub_placement;
+    /* Capacity for active sub-particles; 0 means 8 * num_particles, the
+     * reference allocation (cu:2014-2016). */
+    uint32_t max_sub_particles;
+} sph_params;
+
+/* The three boundary grids of "SPH in complex shapes"
+ * (SphInComplexShapes.cpp:366-427, uploaded as 3D textures by
+ * particle_simulation.cu:2592-2683,2959-3011).  Arrays are x-fastest:
+ * index = i + dims[0]*(j + dims[1]*k).  The lookup follows the reference's
+ * normalized-coordinate, linear-filter, clamp-address texture: a point x is
+ * sampled at texel coordinate ((x-origin)/L)*dims - 0.5 with
+ * L = (dims-1)*spacing. */
+typedef struct sph_boundary {
+    float origin[3];
+    float spacing;
+    int32_t dims[3];
+    float rest_distance;            /* r0 (gBoundaryRestDistance, cu:46) */
+    const float* distance;          /* signed, > 0 inside solids */
+    const float* density;           /* wall density contribution */
+    const float* viscosity;         /* wall viscosity coefficient */
+} sph_boundary;
+
+/* Initial particles plus boundary.  Replaces the host arrays filled by
+ * create_particle_box (cu:2866-2917) and copied by Init (cu:1882-2005). */
+typedef struct sph_scene {
+    uint32_t num_particles;
+    const float* positions;         /* xyz, AoS stride 3 (VD_POS_*, h:17-23) */
+    const float* velocities;        /* xyz, AoS stride 3; NULL = zero */
+    sph_boundary boundary;
+} sph_scene;
+
+typedef struct sph_handle sph_handle;
+
+/* Fields readable with sph_read_field.  Per-particle fields are returned in
+ * original-id order (the reference never permutes its storage, so its
+ * arrays are in that order).  Sorted fields are in cell-sorted order. */
+typedef enum sph_field {
+    SPH_FIELD_POSITION = 0,     /* float xyz [3n]   (VD_POS_*)            */
+    SPH_FIELD_VELOCITY = 1,     /* float xyz [3n]   (SD_VEL0_*)           */
+    SPH_FIELD_DENSITY = 2,      /* float [n]        (SD_DENSITY)          */
+    SPH_FIELD_PRESSURE = 3,     /* float [n]        (SD_PRESSURE)         */
+    SPH_FIELD_ACCEL = 4,        /* float xyz [3n]   (SD_ACC_*)            */
+    SPH_FIELD_STATE = 5,        /* uint8 [n]        (mParticleStates)     */
+    SPH_FIELD_SORTED_KEYS = 6,  /* int32 [n]   (mParticleHashListDevPtr)  */
+    SPH_FIELD_SORTED_IDS = 7,   /* int32 [n]   (mParticleIdsDevPtr)       */
+    SPH_FIELD_CELL_START = 8,   /* int32 [cells]  (mCellStartListDevPtr)  */
+    SPH_FIELD_CELL_END = 9,     /* int32 [cells]  (mCellEndListDevPtr)    */
+    SPH_FIELD_SUB_POSITION = 10,/* float xyz [3*nsub], sub-list order     */
+    SPH_FIELD_SUB_VELOCITY = 11,/* float xyz [3*nsub]                     */
+    SPH_FIELD_SUB_DENSITY = 12, /* float [nsub]                           */
+    SPH_FIELD_SUB_PRESSURE = 13,/* float [nsub]                           */
+    SPH_FIELD_SUB_ACCEL = 14,   /* float xyz [3*nsub]                     */
+    SPH_FIELD_SUB_IDS = 15,     /* int32 [nsub]: 8*parent_id + child      */
+    SPH_FIELD_SPLIT_IDS = 16,   /* int32 [nsplit]  (mParticleIdsSplit)    */
+    SPH_FIELD_BOUNDARY_IDS = 17,/* int32 [nboundary]                      */
+    SPH_FIELD_DEFAULT_IDS = 18, /* int32 [ndefault] (mParticleIdsDefault) */
+    SPH_FIELD_SUB_SORTED_KEYS = 19, /* int32 [nsub] (mSubParticleHashs)   */
+    SPH_FIELD_SUB_SORTED_IDS = 20   /* int32 [nsub] sub ids, sorted       */
+} sph_field;
+
+/* Counts of the two-scale classes (GetNumParticles, GetNumParticlesDefault,
+ * GetNumSubParticles, GetNumSubParticles{Regular,Boundary},
+ * particle_simulation.cu:2227-2316). */
+typedef struct sph_counts {
+    uint32_t num_particles;
+    uint32_t num_split;
+    uint32_t num_boundary;
+    uint32_t num_default;
+    uint32_t num_sub;
+    uint32_t num_steps;             /* GetNumTimesSteps (cu:2245) */
+} sph_counts;
+
+/* ---- lifecycle ---------------------------------------------------------
+ * sph_create replaces the factory + Init + Bind sequence
+ * (particle_simulation.h:174, cu:1745, cu:1882, cu:2078).  Parameters are
+ * validated; the handle owns copies of everything. */
+sph_status sph_create(const sph_params* params, const sph_scene* scene,
+                      sph_handle** out);
+/* ~ParticleSimulation (cu:1689-1735). */
+sph_status sph_destroy(sph_handle* h);
+
+/* ---- stepping ----------------------------------------------------------
+ * sph_step: n_steps of Advance (cu:2085-2117) restricted to the coarse
+ * physics (hash, sort, cell ranges, density/pressure, acceleration,
+ * integration [+ collision]).  Asynchronous on the handle's stream. */
+sph_status sph_step(sph_handle* h, int32_t n_steps);
+/* sph_step_two_scale: n_steps of AdvanceTwoScale (cu:2145-2182): the coarse
+ * step plus state classification, device-side compaction, sub-particle
+ * seeding, projection, sub-grid sort, sub density/acceleration, sub
+ * integration and collision.  No host synchronisation inside. */
+sph_status sph_step_two_scale(sph_handle* h, int32_t n_steps);
+/* Waits for the handle's stream; reports device-side errors. */
+sph_status sph_sync(sph_handle* h);
+
+/* ---- parameters --------------------------------------------------------
+ * Replaces SetNPartThresh / Increase/DecreaseCmDistanceThresh + Bind
+ * (cu:2255-2276).  Takes effect at the next step.  Grid geometry and the
+ * particle count may not change. */
+sph_status sph_set_params(sph_handle* h, const sph_params* params);
+sph_status sph_get_params(const sph_handle* h, sph_params* out);
+
+/* ---- read back / map ---------------------------------------------------
+ * sph_read_positions replaces the GL-mapped VBO read
+ * (GetGLParticleVertexBufferObject + map, cu:2278,2685): xyz in
+ * original-id order. */
+sph_status sph_read_positions(sph_handle* h, float* xyz, size_t n);
+/* Generic field read (GetParticleState cu:2238, SaveInfoTable cu:2709,
+ * SaveParticleInfo cu:2785).  bytes must match the field size exactly. */
+sph_status sph_read_field(sph_handle* h, sph_field field, void* dst,
+                          size_t bytes);
+/* Zero-copy access: device pointers to the cell-sorted float4 positions
+ * (x, y, z, density) and the original id of each sorted slot.  Valid until
+ * the next step or destroy. */
+sph_status sph_map_positions(sph_handle* h, const float** dev_pos4,
+                             const uint32_t** dev_orig_id);
+sph_status sph_get_counts(sph_handle* h, sph_counts* out);
+size_t sph_num_cells(const sph_handle* h);
+/* GetSizeMemoryGPU (particle_simulation.h:169; not defined in the
+ * reference): device bytes owned by the handle. */
+size_t sph_device_bytes(const sph_handle* h);
+/* Message of the last error on this handle ("" if none); NULL handle gives
+ * the last error of sph_create. */
+const char* sph_last_error_message(const sph_handle* h);
+
+/* ---- timing ------------------------------------------------------------
+ * Per-stage device time of the last sph_step call, measured with HIP events
+ * on the handle's stream when profiling is enabled (CudaTimer,
+ * util.h:193-213, re-expressed).  Stage names: see sph_stage_name. */
+#define SPH_MAX_STAGES 16
+sph_status sph_enable_timing(sph_handle* h, int32_t enable);
+sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count);
+const char* sph_stage_name(int32_t stage);
+/* Device time of n steps measured with HIP events recorded on the handle's
+ * stream around the steps (no host work inside the timed region). */
+sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
+                          float* ms);
+
+/* ---- batched scenes ----------------------------------------------------
+ * Many independent scenes advanced together: one launch per stage over all
+ * scenes, scenes kept in disjoint key ranges of one segmented sort.  The
+ * per-scene interface is that of sph_create.  No reference counterpart (the
+ * reference binds one simulation at a time, particle_simulation.h:122-125);
+ * this generalises sph_create/sph_step to a list of scenes. */
+typedef struct sph_batch sph_batch;
+sph_status sph_batch_create(int32_t n_scenes, const sph_params* params,
+                            const sph_scene* scenes, sph_batch** out);
+sph_status sph_batch_destroy(sph_batch* b);
+sph_status sph_batch_step(sph_batch* b, int32_t n_steps);
+sph_status sph_batch_sync(sph_batch* b);
+/* Positions of scene s in that scene's original-id order. */
+sph_status sph_batch_read_field(sph_batch* b, int32_t scene, sph_field field,
+                                void* dst, size_t bytes);
+sph_status sph_batch_time_steps(sph_batch* b, int32_t n_steps, float* ms);
+/* hipGraph capture of one batched step (0 = launch kernels directly). */
+sph_status sph_batch_use_graph(sph_batch* b, int32_t enable);
+uint64_t sph_batch_num_particles(const sph_batch* b);
+const char* sph_batch_last_error_message(const sph_batch* b);
+
+/* ---- scene construction (host only, no device needed) ------------------
+ * Replaces Example01 (cu:1745-1817), create_particle_box (cu:2866-2917),
+ * set_simulation_domain (cu:2919-2941) and the boundary grid builder
+ * SphInComplexShapes (SphInComplexShapes.cpp:34-64,203-313,366-427). */
+#define SPH_MAX_OBSTACLES 8
+typedef enum sph_solid_kind {
+    SPH_SOLID_NONE = 0,
+    SPH_SOLID_SPHERE = 1,   /* center, radius = a[0]                     */
+    SPH_SOLID_CONE = 2,     /* base center, base radius a[0], height a[1],
+                               apex at center + (0, a[1], 0)             */
+    SPH_SOLID_BOX = 3       /* center, half extents a[0..2]              */
+} sph_solid_kind;
+
+typedef struct sph_solid {
+    int32_t kind;
+    float center[3];
+    float a[3];
+} sph_solid;
+
+typedef struct sph_scene_desc {
+    /* fluid block: lattice[0]*lattice[1]*lattice[2] particles at
+     * origin + spacing*(i,j,k), jittered by U(-jitter, jitter) per axis
+     * from a counter-based hash of (seed, particle, axis). */
+    uint32_t lattice[3];
+    float lattice_origin[3];
+    float lattice_spacing;
+    float jitter;
+    uint64_t seed;
+    /* container box (fluid inside), obstacles (solid) */
+    float box_lo[3];
+    float box_hi[3];
+    int32_t num_obstacles;
+    sph_solid obstacles[SPH_MAX_OBSTACLES];
+    /* physics */
+    float h;
+    float rest_density;
+    float particle_mass;
+    float gas_stiffness;
+    float viscosity;
+    float gravity;
+    float tension;
+    float norm_thresh;
+    float dt;
+    float dt_sub;
+    /* cell grid: set_simulation_domain(cell_lo, cell_hi, h, h/2)
+     * (cu:2919-2941): dims = int((hi-lo)/spacing + 0.5) */
+    float cell_lo[3];
+    float cell_hi[3];
+    /* boundary grid */
+    float boundary_origin[3];
+    float boundary_spacing;
+    int32_t boundary_dims[3];
+    float rest_distance;            /* r0 */
+    float boundary_support;         /* cs of the grid builder */
+    float ghost_spacing;            /* 0 = (m/rho0)^(1/3) (cu:2598) */
+    /* options copied into sph_params */
+    int32_t collision;
+    float restitution;
+    float split_lo[3];
+    float split_hi[3];
+    int32_t sub_placement;
+} sph_scene_desc;
+
+/* Named workloads. */
+typedef enum sph_config_id {
+    SPH_CONFIG_EXAMPLE01 = 0,   /* particle_simulation.cu:1745-1817 */
+    SPH_CONFIG_DAM16K = 1,      /* 16,384 particles, 0.64^3 box, 64^3 grid */
+    SPH_CONFIG_DAM1M = 2,       /* 1,048,576 particles, sphere + cone */
+    SPH_CONFIG_TWO_SCALE = 3,   /* 262,144 coarse + sub-particles */
+    SPH_CONFIG_BATCH = 5        /* one scene of the 64-scene batch */
+} sph_config_id;
+
+/* Fills desc for a named workload.  For SPH_CONFIG_BATCH, index selects the
+ * scene (0..63) of the batch drawn from seed. */
+sph_status sph_scene_config(int32_t config, int32_t index, uint64_t seed,
+                            sph_scene_desc* desc);
+/* Derives sph_params (kernel coefficients, grids) from a description. */
+sph_status sph_scene_params(const sph_scene_desc* desc, sph_params* out);
+/* Generates particle positions (xyz AoS).  Returns the count in *n; pass
+ * pos = NULL to query the count. */
+sph_status sph_scene_particles(const sph_scene_desc* desc, float* pos,
+                               uint32_t capacity, uint32_t* n);
+/* Builds the three boundary grids (dims from desc).  Follows
+ * SphInComplexShapes::SetBox for the container box and extends the same
+ * ghost-particle construction to the obstacle solids.  Each output holds
+ * boundary_dims[0]*[1]*[2] floats. */
+sph_status sph_scene_boundary(const sph_scene_desc* desc, float* distance,
+                              float* density, float* viscosity);
+/* Signed distance of the union of solids at n points (FP32 analytic),
+ * > 0 inside a solid. */
+sph_status sph_scene_sdf(const sph_scene_desc* desc, const float* xyz,
+                         size_t n, float* out);
+
+/* Example01's own grid builder call (cu:2592-2612): box of half extents
+ * (0.7,0.5,0.3) on the grid (-0.8,-0.6,-0.4)..(0.8,0.6,0.4), spacing h/4.
+ * Writes dims and, if the pointers are non-NULL, the node data. */
+sph_status sph_example01_boundary(int32_t dims[3], float origin[3],
+                                  float* spacing, float* rest_distance,
+                                  float* distance, float* density,
+                                  float* viscosity);
+
+/* ---- library info ----------------------------------------------------- */
+int32_t sph_abi_version(void);
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int32_t sph_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPH_H */

@@ -1,0 +1,196 @@
+"""ctypes wrapper of the CPU oracle liboracle_sph.so -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+use this module, as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from cudasphnew_amd._native import Field, SceneDesc, SphBoundary, SphCounts, SphParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_sph.so")
+
+FLAG_OUT_OF_DOMAIN = 1
+FLAG_NONFINITE = 2
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} missing; run `make -C oracle`")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, fp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    sig = {
+        "orc_create": (vp, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, fp, fp, C.c_int, C.c_int]),
+        "orc_create_f64": (vp, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, dp, dp, C.c_int,
+                                C.c_int]),
+        "orc_destroy": (None, [vp]),
+        "orc_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "orc_read": (C.c_long, [vp, C.c_int, vp, C.c_size_t]),
+        "orc_counts": (None, [vp, C.POINTER(SphCounts)]),
+        "orc_flags": (C.c_uint32, [vp]),
+        "orc_last_step_seconds": (C.c_double, [vp]),
+        "orc_keys": (C.c_uint32, [C.POINTER(SphParams), C.c_uint32, fp, ip, C.c_int]),
+        "orc_sort_cells": (None, [C.POINTER(SphParams), C.c_uint32, ip, ip, ip, ip, ip]),
+        "orc_tex": (None, [C.POINTER(SphBoundary), C.c_int, C.c_uint32, dp, dp]),
+        "orc_density_brute": (None, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, dp, dp]),
+        "orc_accel_brute": (None, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, dp, dp, dp, dp]),
+        "orc_scene_boundary": (C.c_int, [C.POINTER(SceneDesc), dp, dp, dp]),
+        "orc_scene_sdf": (None, [C.POINTER(SceneDesc), C.c_uint32, dp, dp]),
+        "orc_struct_layout": (None, [C.POINTER(C.c_size_t)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class OracleSim:
+    """Stateful oracle simulation in original-id order.  real=0: FP64
+    oracle, real=1: FP32 transcription (CPU baseline)."""
+
+    def __init__(self, scene, real: int = 0, threads: int = 1, positions=None, velocities=None):
+        self.scene = scene
+        pos = scene.positions if positions is None else positions
+        vel = scene.velocities if velocities is None else velocities
+        self._b = scene.sph_boundary()
+        self._p = SphParams.from_buffer_copy(scene.params)
+        self._p.num_particles = pos.shape[0]
+        self.n = pos.shape[0]
+        if pos.dtype == np.float64:
+            pd = np.ascontiguousarray(pos, np.float64)
+            vd = None if vel is None else np.ascontiguousarray(vel, np.float64)
+            self._s = lib.orc_create_f64(C.byref(self._p), C.byref(self._b), self.n, _dp(pd),
+                                         None if vd is None else _dp(vd), real, threads)
+        else:
+            pf = np.ascontiguousarray(pos, np.float32)
+            vf = None if vel is None else np.ascontiguousarray(vel, np.float32)
+            self._s = lib.orc_create(C.byref(self._p), C.byref(self._b), self.n, _fp(pf),
+                                     None if vf is None else _fp(vf), real, threads)
+        if not self._s:
+            raise RuntimeError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib.orc_destroy(self._s)
+            self._s = None
+
+    def step(self, n: int = 1, two_scale: bool = False) -> float:
+        lib.orc_step(self._s, n, 1 if two_scale else 0)
+        return float(lib.orc_last_step_seconds(self._s))
+
+    def counts(self) -> SphCounts:
+        c = SphCounts()
+        lib.orc_counts(self._s, C.byref(c))
+        return c
+
+    def flags(self) -> int:
+        return int(lib.orc_flags(self._s))
+
+    def read(self, field) -> np.ndarray:
+        field = Field(field)
+        c = self.counts()
+        n, nsub = self.n, c.num_sub
+        ncells = int(np.prod([self._p.grid_dim[a] for a in range(3)]))
+        shapes = {
+            Field.POSITION: ((n, 3), np.float64), Field.VELOCITY: ((n, 3), np.float64),
+            Field.ACCEL: ((n, 3), np.float64), Field.DENSITY: ((n,), np.float64),
+            Field.PRESSURE: ((n,), np.float64), Field.STATE: ((n,), np.uint8),
+            Field.SORTED_KEYS: ((n,), np.int32), Field.SORTED_IDS: ((n,), np.int32),
+            Field.CELL_START: ((ncells,), np.int32), Field.CELL_END: ((ncells,), np.int32),
+            Field.SUB_POSITION: ((nsub, 3), np.float64), Field.SUB_VELOCITY: ((nsub, 3), np.float64),
+            Field.SUB_ACCEL: ((nsub, 3), np.float64), Field.SUB_DENSITY: ((nsub,), np.float64),
+            Field.SUB_PRESSURE: ((nsub,), np.float64), Field.SUB_IDS: ((nsub,), np.int32),
+            Field.SPLIT_IDS: ((c.num_split,), np.int32), Field.BOUNDARY_IDS: ((c.num_boundary,), np.int32),
+            Field.DEFAULT_IDS: ((c.num_default,), np.int32), Field.SUB_SORTED_KEYS: ((nsub,), np.int32),
+            Field.SUB_SORTED_IDS: ((nsub,), np.int32),
+        }
+        shape, dt = shapes[field]
+        out = np.empty(shape, dt)
+        got = lib.orc_read(self._s, int(field), out.ctypes.data_as(C.c_void_p), out.nbytes)
+        if got < 0:
+            raise RuntimeError(f"orc_read({field.name}) failed")
+        return out
+
+
+def keys(params: SphParams, pos: np.ndarray, real: int = 1):
+    pf = np.ascontiguousarray(pos, np.float32)
+    out = np.empty(pf.shape[0], np.int32)
+    flags = lib.orc_keys(C.byref(params), pf.shape[0], _fp(pf), _ip(out), real)
+    return out, int(flags)
+
+
+def sort_cells(params: SphParams, keys_: np.ndarray):
+    k = np.ascontiguousarray(keys_, np.int32)
+    n = k.shape[0]
+    nc = int(np.prod([params.grid_dim[a] for a in range(3)]))
+    sk, si = np.empty(n, np.int32), np.empty(n, np.int32)
+    cs, ce = np.empty(nc, np.int32), np.empty(nc, np.int32)
+    lib.orc_sort_cells(C.byref(params), n, _ip(k), _ip(sk), _ip(si), _ip(cs), _ip(ce))
+    return sk, si, cs, ce
+
+
+def tex(scene, which: int, xyz: np.ndarray) -> np.ndarray:
+    b = scene.sph_boundary()
+    p = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    out = np.empty(p.shape[0], np.float64)
+    lib.orc_tex(C.byref(b), which, p.shape[0], _dp(p), _dp(out))
+    return out
+
+
+def density_brute(scene, pos: np.ndarray) -> np.ndarray:
+    b = scene.sph_boundary()
+    p = np.ascontiguousarray(pos, np.float64)
+    out = np.empty(p.shape[0], np.float64)
+    lib.orc_density_brute(C.byref(scene.params), C.byref(b), p.shape[0], _dp(p), _dp(out))
+    return out
+
+
+def accel_brute(scene, pos, vel, rho) -> np.ndarray:
+    b = scene.sph_boundary()
+    p = np.ascontiguousarray(pos, np.float64)
+    v = np.ascontiguousarray(vel, np.float64)
+    r = np.ascontiguousarray(rho, np.float64)
+    out = np.empty_like(p)
+    lib.orc_accel_brute(C.byref(scene.params), C.byref(b), p.shape[0], _dp(p), _dp(v), _dp(r), _dp(out))
+    return out
+
+
+def scene_boundary(desc: SceneDesc):
+    nx, ny, nz = (int(v) for v in desc.boundary_dims)
+    out = [np.empty((nz, ny, nx), np.float64) for _ in range(3)]
+    lib.orc_scene_boundary(C.byref(desc), *(_dp(a) for a in out))
+    return out
+
+
+def scene_sdf(desc: SceneDesc, xyz: np.ndarray) -> np.ndarray:
+    p = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    out = np.empty(p.shape[0], np.float64)
+    lib.orc_scene_sdf(C.byref(desc), p.shape[0], _dp(p), _dp(out))
+    return out
+
+
+def struct_layout():
+    out = (C.c_size_t * 10)()
+    lib.orc_struct_layout(out)
+    return list(out)
