@@ -153,7 +153,7 @@ class SphError(RuntimeError):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
-            f"{LIB_PATH} is missing; build it with `python -m cudasphnew_amd.build`")
+            f"{LIB_PATH} is missing; build it with `python cudasphnew_amd/build.py`")
     return C.CDLL(LIB_PATH)
 
 
@@ -187,6 +187,8 @@ SIGNATURES = [
     ("sph_batch_sync", _st, [_vp]),
     ("sph_batch_read_field", _st, [_vp, C.c_int32, C.c_int, _vp, C.c_size_t]),
     ("sph_batch_time_steps", _st, [_vp, C.c_int32, fptr]),
+    ("sph_batch_enable_timing", _st, [_vp, C.c_int32]),
+    ("sph_batch_stage_times", _st, [_vp, fptr, C.POINTER(C.c_int32)]),
     ("sph_batch_use_graph", _st, [_vp, C.c_int32]),
     ("sph_batch_num_particles", C.c_uint64, [_vp]),
     ("sph_batch_last_error_message", C.c_char_p, [_vp]),

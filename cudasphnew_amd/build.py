@@ -1,7 +1,9 @@
 """Build recipe of the HIP library libsph_hip.so (gfx950) and the CPU oracle.
 
-    python -m cudasphnew_amd.build            # library + oracle
-    python -m cudasphnew_amd.build --lib-only
+    python cudasphnew_amd/build.py            # library + oracle
+    python cudasphnew_amd/build.py --lib-only
+
+(run as a script: importing the package would load the library being built)
 
 Objects go to cudasphnew_amd/_build/, the library to
 cudasphnew_amd/libsph_hip.so (in-tree, so that it travels with the source

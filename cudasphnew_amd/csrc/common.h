@@ -130,6 +130,12 @@ __device__ __forceinline__ TexCoord tex_setup(const DevScene& S, float tx, float
     return c;
 }
 
+// a + t (b - a): reproduces a constant field exactly, so the clamped
+// "far from any wall" value -r0 interpolates to exactly -r0 and the
+// wall-term switch distWall < r0 (particle_simulation.cu:422) is not flipped
+// by rounding.
+__device__ __forceinline__ float lerp1(float a, float b, float t) { return a + t * (b - a); }
+
 template <class T>
 __device__ __forceinline__ T lerp3(const T* __restrict__ g, const DevScene& S, const TexCoord& c);
 
@@ -142,18 +148,13 @@ __device__ __forceinline__ float lerp3<float>(const float* __restrict__ g, const
     float v010 = b[c.x0 * sx + c.y1 * sy + c.z0 * sz], v110 = b[c.x1 * sx + c.y1 * sy + c.z0 * sz];
     float v001 = b[c.x0 * sx + c.y0 * sy + c.z1 * sz], v101 = b[c.x1 * sx + c.y0 * sy + c.z1 * sz];
     float v011 = b[c.x0 * sx + c.y1 * sy + c.z1 * sz], v111 = b[c.x1 * sx + c.y1 * sy + c.z1 * sz];
-    float c00 = v000 * (1.0f - c.ax) + v100 * c.ax;
-    float c10 = v010 * (1.0f - c.ax) + v110 * c.ax;
-    float c01 = v001 * (1.0f - c.ax) + v101 * c.ax;
-    float c11 = v011 * (1.0f - c.ax) + v111 * c.ax;
-    float c0 = c00 * (1.0f - c.ay) + c10 * c.ay;
-    float c1 = c01 * (1.0f - c.ay) + c11 * c.ay;
-    return c0 * (1.0f - c.az) + c1 * c.az;
+    float c00 = lerp1(v000, v100, c.ax), c10 = lerp1(v010, v110, c.ax);
+    float c01 = lerp1(v001, v101, c.ax), c11 = lerp1(v011, v111, c.ax);
+    return lerp1(lerp1(c00, c10, c.ay), lerp1(c01, c11, c.ay), c.az);
 }
 
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
-    float s = 1.0f - t;
-    return make_float4(a.x * s + b.x * t, a.y * s + b.y * t, a.z * s + b.z * t, a.w * s + b.w * t);
+    return make_float4(lerp1(a.x, b.x, t), lerp1(a.y, b.y, t), lerp1(a.z, b.z, t), lerp1(a.w, b.w, t));
 }
 
 template <>

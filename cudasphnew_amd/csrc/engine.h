@@ -68,13 +68,18 @@ struct Engine {
     uint32_t steps = 0;
     std::string err;
 
-    // timing
+    // timing: events recorded on the stream before each step and after
+    // each stage (stage -1 marks a step start); resolved without per-step
+    // host synchronisation when the times are requested.
     bool timing = false;
-    hipEvent_t ev[kNumStages + 1] = {};
-    float stage_ms[kNumStages] = {};
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, hipEvent_t>> ev_rec;
     float stage_acc_ms[kNumStages] = {};
+    uint32_t stage_launches[kNumStages] = {};
     uint32_t stage_steps = 0;
-    std::vector<int> recorded;   // stages marked during the last enqueue
+    hipEvent_t next_event();
+    sph_status resolve_timing();
 
     // graph capture of the steady-state step
     bool use_graph = false;

@@ -104,8 +104,7 @@ Engine::~Engine() {
     if (ts) twoscale_destroy(*this);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (int i = 0; i <= kNumStages; ++i)
-        if (ev[i]) (void)hipEventDestroy(ev[i]);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
                     perm, slot_of_oid, state_a, state_b, range, status, n_dev};
     for (void* p : ptrs)
@@ -258,8 +257,6 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
         hipMemset(skey, 0, N * sizeof(uint32_t)) != hipSuccess ||
         hipMemset(oid_b, 0, N * sizeof(uint32_t)) != hipSuccess)
         return fail(SPH_DEVICE_ERROR, "initial clear failed");
-    for (int i = 0; i <= kNumStages; ++i)
-        if (hipEventCreate(&ev[i]) != hipSuccess) return fail(SPH_DEVICE_ERROR, "hipEventCreate failed");
     if (hipDeviceSynchronize() != hipSuccess) return fail(SPH_DEVICE_ERROR, "device sync after create failed");
 #undef SPH_TRY
     return SPH_OK;
@@ -285,16 +282,47 @@ StepBuffers Engine::buffers() const {
     return b;
 }
 
+hipEvent_t Engine::next_event() {
+    if (ev_used == ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+}
+
 void Engine::mark(int stage) {
     if (!timing) return;
-    (void)hipEventRecord(ev[stage + 1], stream);
-    recorded.push_back(stage);
+    hipEvent_t e = next_event();
+    if (!e) return;
+    (void)hipEventRecord(e, stream);
+    ev_rec.emplace_back(stage, e);
+}
+
+sph_status Engine::resolve_timing() {
+    if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "timing sync failed");
+    hipEvent_t prev = nullptr;
+    for (auto& r : ev_rec) {
+        if (r.first < 0) {
+            prev = r.second;
+            ++stage_steps;
+            continue;
+        }
+        float ms = 0.0f;
+        if (prev && hipEventElapsedTime(&ms, prev, r.second) == hipSuccess) {
+            stage_acc_ms[r.first] += ms;
+            stage_launches[r.first] += 1;
+        }
+        prev = r.second;
+    }
+    ev_rec.clear();
+    ev_used = 0;
+    return SPH_OK;
 }
 
 void Engine::enqueue_step(bool two_scale) {
     StepBuffers b = buffers();
-    recorded.clear();
-    if (timing) (void)hipEventRecord(ev[0], stream);
+    mark(-1);
     if (!keys_valid) launch_hash(stream, b);
     mark(kStageHash);
     if (cells_valid) launch_cells_clear(stream, b);
@@ -336,19 +364,10 @@ sph_status Engine::step(int n, bool two_scale) {
                 return fail(SPH_DEVICE_ERROR, "hipGraphLaunch failed");
         } else {
             enqueue_step(two_scale);
-            if (timing) {
-                // stage durations: consecutive events in enqueue order
-                hipEvent_t last = ev[recorded.back() + 1];
-                if (hipEventSynchronize(last) != hipSuccess) return fail(SPH_DEVICE_ERROR, "timing sync failed");
-                hipEvent_t prev = ev[0];
-                for (int s : recorded) {
-                    float ms = 0.0f;
-                    (void)hipEventElapsedTime(&ms, prev, ev[s + 1]);
-                    stage_ms[s] = ms;
-                    stage_acc_ms[s] += ms;
-                    prev = ev[s + 1];
-                }
-                ++stage_steps;
+            // bound the number of outstanding timing events
+            if (timing && ev_rec.size() > 4096) {
+                sph_status st = resolve_timing();
+                if (st != SPH_OK) return st;
             }
         }
         sph_status st = check_launch("step");
@@ -617,21 +636,32 @@ const char* sph_last_error_message(const sph_handle* h) {
     return h ? h->e.err.c_str() : g_create_error.c_str();
 }
 
-sph_status sph_enable_timing(sph_handle* h, int32_t enable) {
-    if (!h) return SPH_INVALID_ARG;
-    h->e.timing = enable != 0;
-    std::memset(h->e.stage_acc_ms, 0, sizeof(h->e.stage_acc_ms));
-    h->e.stage_steps = 0;
+static sph_status enable_timing(Engine& e, int32_t enable) {
+    sph_status st = e.resolve_timing();
+    e.timing = enable != 0;
+    std::memset(e.stage_acc_ms, 0, sizeof(e.stage_acc_ms));
+    std::memset(e.stage_launches, 0, sizeof(e.stage_launches));
+    e.stage_steps = 0;
+    return st;
+}
+
+static sph_status stage_times(Engine& e, float* ms, int32_t* count) {
+    if (!ms || !count) return SPH_INVALID_ARG;
+    sph_status st = e.resolve_timing();
+    if (st != SPH_OK) return st;
+    // average device time per step of each stage since timing was enabled
+    uint32_t n = e.stage_steps ? e.stage_steps : 1;
+    for (int s = 0; s < sph::kNumStages && s < SPH_MAX_STAGES; ++s) ms[s] = e.stage_acc_ms[s] / (float)n;
+    *count = sph::kNumStages < SPH_MAX_STAGES ? sph::kNumStages : SPH_MAX_STAGES;
     return SPH_OK;
 }
 
+sph_status sph_enable_timing(sph_handle* h, int32_t enable) {
+    return h ? enable_timing(h->e, enable) : SPH_INVALID_ARG;
+}
+
 sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count) {
-    if (!h || !ms || !count) return SPH_INVALID_ARG;
-    // average per step since timing was enabled
-    uint32_t n = h->e.stage_steps ? h->e.stage_steps : 1;
-    for (int s = 0; s < sph::kNumStages && s < SPH_MAX_STAGES; ++s) ms[s] = h->e.stage_acc_ms[s] / (float)n;
-    *count = sph::kNumStages < SPH_MAX_STAGES ? sph::kNumStages : SPH_MAX_STAGES;
-    return SPH_OK;
+    return h ? stage_times(h->e, ms, count) : SPH_INVALID_ARG;
 }
 
 const char* sph_stage_name(int32_t stage) { return sph::stage_name(stage); }
@@ -695,6 +725,14 @@ sph_status sph_batch_read_field(sph_batch* b, int32_t scene, sph_field f, void* 
 
 sph_status sph_batch_time_steps(sph_batch* b, int32_t n, float* ms) {
     return b ? time_engine(b->e, n, false, ms) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_enable_timing(sph_batch* b, int32_t enable) {
+    return b ? enable_timing(b->e, enable) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_stage_times(sph_batch* b, float* ms, int32_t* count) {
+    return b ? stage_times(b->e, ms, count) : SPH_INVALID_ARG;
 }
 
 sph_status sph_batch_use_graph(sph_batch* b, int32_t enable) {

@@ -225,9 +225,11 @@ size_t sph_device_bytes(const sph_handle* h);
 const char* sph_last_error_message(const sph_handle* h);
 
 /* ---- timing ------------------------------------------------------------
- * Per-stage device time of the last sph_step call, measured with HIP events
- * on the handle's stream when profiling is enabled (CudaTimer,
- * util.h:193-213, re-expressed).  Stage names: see sph_stage_name. */
+ * Per-stage device time, averaged per step since sph_enable_timing, measured
+ * with HIP events recorded on the handle's stream around every stage
+ * (CudaTimer, util.h:193-213, re-expressed); the events are resolved when
+ * sph_stage_times is called, so timing adds no host synchronisation to a
+ * step.  Stage names: see sph_stage_name. */
 #define SPH_MAX_STAGES 16
 sph_status sph_enable_timing(sph_handle* h, int32_t enable);
 sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count);
@@ -253,6 +255,9 @@ sph_status sph_batch_sync(sph_batch* b);
 sph_status sph_batch_read_field(sph_batch* b, int32_t scene, sph_field field,
                                 void* dst, size_t bytes);
 sph_status sph_batch_time_steps(sph_batch* b, int32_t n_steps, float* ms);
+/* Per-stage device times of a batch, as sph_enable_timing/sph_stage_times. */
+sph_status sph_batch_enable_timing(sph_batch* b, int32_t enable);
+sph_status sph_batch_stage_times(sph_batch* b, float* ms, int32_t* count);
 /* hipGraph capture of one batched step (0 = launch kernels directly). */
 sph_status sph_batch_use_graph(sph_batch* b, int32_t enable);
 uint64_t sph_batch_num_particles(const sph_batch* b);

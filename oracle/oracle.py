@@ -17,6 +17,8 @@ LIB_PATH = os.path.join(HERE, "liboracle_sph.so")
 
 FLAG_OUT_OF_DOMAIN = 1
 FLAG_NONFINITE = 2
+FIELD_COLOR_GRAD_NORM = 100
+FIELD_WALL_DISTANCE = 101
 
 
 def _load():
@@ -106,6 +108,23 @@ class OracleSim:
 
     def flags(self) -> int:
         return int(lib.orc_flags(self._s))
+
+    def branch_margin(self) -> np.ndarray:
+        """Relative distance of each particle from the two switches of the
+        acceleration (|grad c| vs normThresh, wall distance vs r0)."""
+        out = []
+        for f in (FIELD_COLOR_GRAD_NORM, FIELD_WALL_DISTANCE):
+            a = np.empty(self.n, np.float64)
+            if lib.orc_read(self._s, f, a.ctypes.data_as(C.c_void_p), a.nbytes) < 0:
+                raise RuntimeError("orc_read failed")
+            out.append(a)
+        cg, dw = out
+        thr = float(self._p.norm_thresh)
+        r0 = float(self._b.rest_distance)
+        # distWall == r0 exactly (constant clamped field, interpolated
+        # exactly) is not ambiguous: both precisions skip the wall term
+        wall = np.where(dw == r0, np.inf, np.abs(dw / r0 - 1.0))
+        return np.minimum(np.abs(cg / thr - 1.0), wall)
 
     def read(self, field) -> np.ndarray:
         field = Field(field)

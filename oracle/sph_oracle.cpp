@@ -170,13 +170,13 @@ R tex3d(const Grid& g, int which, R u, R v, R w) {
     auto at = [&](int i, int j, int k) -> R {
         return (R)T[(size_t)i + (size_t)g.dims[0] * ((size_t)j + (size_t)g.dims[1] * k)];
     };
-    R c00 = at(i0[0], i0[1], i0[2]) * (1 - al[0]) + at(i1[0], i0[1], i0[2]) * al[0];
-    R c10 = at(i0[0], i1[1], i0[2]) * (1 - al[0]) + at(i1[0], i1[1], i0[2]) * al[0];
-    R c01 = at(i0[0], i0[1], i1[2]) * (1 - al[0]) + at(i1[0], i0[1], i1[2]) * al[0];
-    R c11 = at(i0[0], i1[1], i1[2]) * (1 - al[0]) + at(i1[0], i1[1], i1[2]) * al[0];
-    R c0 = c00 * (1 - al[1]) + c10 * al[1];
-    R c1 = c01 * (1 - al[1]) + c11 * al[1];
-    return c0 * (1 - al[2]) + c1 * al[2];
+    // a + t (b - a): exact for a constant field (see the wall switch)
+    auto lerp = [](R a, R b, R t) { return a + t * (b - a); };
+    R c00 = lerp(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), al[0]);
+    R c10 = lerp(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), al[0]);
+    R c01 = lerp(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), al[0]);
+    R c11 = lerp(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), al[0]);
+    return lerp(lerp(c00, c10, al[1]), lerp(c01, c11, al[1]), al[2]);
 }
 
 // Normalized texture coordinates of a world point (cu:243-245).
@@ -200,6 +200,10 @@ struct Sim {
     size_t ncells = 0, nsubcells = 0;
 
     std::vector<R> x, v, a, rho, p;          // 3N, 3N, 3N, N, N
+    // branch variables of the acceleration (colour-gradient norm against
+    // normThresh, wall distance against r0), for tests that must skip
+    // particles sitting on a discontinuity of the reference's model
+    std::vector<double> cgn_out, dwall_out;
     std::vector<uint8_t> state;              // N
     std::vector<int32_t> keys, skeys, sids, cstart, cend;
 
@@ -220,6 +224,8 @@ struct Sim {
         v.assign(3 * (size_t)n, (R)0);
         if (vel) v.assign(vel, vel + 3 * (size_t)n);
         a.assign(3 * (size_t)n, (R)0);
+        cgn_out.assign(n, 0.0);
+        dwall_out.assign(n, 0.0);
         rho.assign(n, (R)0);
         p.assign(n, (R)0);
         state.assign(n, 0);
@@ -316,6 +322,8 @@ struct Sim {
             tex_coords(G, xi, uvw);
             R distWall = -tex3d<R>(G, 0, uvw[0], uvw[1], uvw[2]);
             R r0 = (R)G.r0;
+            dwall_out[id] = (double)distWall;
+            cgn_out[id] = std::sqrt((double)cg[0] * cg[0] + (double)cg[1] * cg[1] + (double)cg[2] * cg[2]);
             if (distWall < r0) {
                 R coeff = rhoi / (dt * dt) * (r0 - distWall);
                 R gn[3];
@@ -639,6 +647,11 @@ struct Sim {
             for (size_t i = 0; i < n; ++i) o[i] = (double)src[i];
             return (long)n;
         };
+        auto put_dbl = [&](const std::vector<double>& src, size_t n) -> long {
+            if (bytes < n * sizeof(double)) return -1;
+            if (n) std::memcpy(dst, src.data(), n * sizeof(double));
+            return (long)n;
+        };
         auto put_int = [&](const int32_t* src, size_t n) -> long {
             if (bytes < n * sizeof(int32_t)) return -1;
             if (n) std::memcpy(dst, src, n * sizeof(int32_t));
@@ -668,6 +681,8 @@ struct Sim {
         case SPH_FIELD_BOUNDARY_IDS: return put_int(boundary_ids.data(), nboundary);
         case SPH_FIELD_DEFAULT_IDS: return put_int(default_ids.data(), ndefault);
         case SPH_FIELD_SUB_SORTED_KEYS: return put_int(sub_skeys.data(), nsub);
+        case ORC_FIELD_COLOR_GRAD_NORM: return put_dbl(cgn_out, N);
+        case ORC_FIELD_WALL_DISTANCE: return put_dbl(dwall_out, N);
         case SPH_FIELD_SUB_SORTED_IDS: {
             if (bytes < nsub * sizeof(int32_t)) return -1;
             int32_t* o = (int32_t*)dst;

@@ -26,6 +26,12 @@ extern "C" {
 #define ORC_FLAG_OUT_OF_DOMAIN 1u
 #define ORC_FLAG_NONFINITE 2u
 
+/* oracle-only fields (double [n]) of the last acceleration pass: the
+ * colour-gradient norm compared with normThresh and the wall distance
+ * compared with r0 (the two switches of particle_simulation.cu:407,457) */
+#define ORC_FIELD_COLOR_GRAD_NORM 100
+#define ORC_FIELD_WALL_DISTANCE 101
+
 typedef struct orc_sim orc_sim;
 
 /* ---- stateful simulation (storage in original-id order, as the
