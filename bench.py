@@ -96,6 +96,8 @@ def main(argv=None) -> int:
     ap.add_argument("--config", default="dam1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -104,6 +106,13 @@ def main(argv=None) -> int:
 
     import cudasphnew_amd as S
 
+    if a.copy_probe:
+        import torch   # plumbing only: a copy kernel of known size
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read, 268435456 bytes written")
     t0 = time.time()
     sc = S.make_scene(a.config)
     log(f"scene {a.config}: {sc.n} particles, grid {tuple(sc.params.grid_dim)}, "

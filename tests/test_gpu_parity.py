@@ -165,7 +165,8 @@ def test_single_particle_free_fall(dam16k):
     # explicit Euler would give n(n-1)/2: 1e-4 m away; FP32 rounding of
     # 1000 position updates stays below 2e-5 m
     assert sim.positions()[0, 1] == pytest.approx(y, abs=2e-5)
-    assert sim.read(Field.VELOCITY)[0, 1] == pytest.approx(-G * dt * n, rel=1e-5)
+    # 1000 FP32 additions of the same increment: rounding accumulates linearly
+    assert sim.read(Field.VELOCITY)[0, 1] == pytest.approx(-G * dt * n, rel=5e-5)
 
 
 def test_dense_cluster_long_cell_runs(dam16k):
