@@ -112,5 +112,6 @@ void twoscale_enqueue(Engine& e);
 sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
 sph_status twoscale_counts(Engine& e, sph_counts* out);
 size_t twoscale_bytes(const Engine& e);
+uint8_t* twoscale_final_state(Engine& e);
 
 }  // namespace sph

@@ -276,7 +276,7 @@ StepBuffers Engine::buffers() const {
     b.key = key;
     b.skey = skey;
     b.perm = perm;
-    b.slot_of_oid = slot_of_oid;
+    b.slot_of_oid = ts ? slot_of_oid : nullptr;
     b.range = range;
     b.status = status;
     return b;
@@ -335,7 +335,10 @@ void Engine::enqueue_step(bool two_scale) {
     mark(kStageReorder);
     launch_density(stream, b, 256);
     mark(kStageDensity);
-    if (two_scale) twoscale_enqueue(*this);
+    if (two_scale) {
+        twoscale_enqueue(*this);
+        b.state_b = twoscale_final_state(*this);
+    }
     launch_force(stream, b, 256);
     mark(kStageForce);
     keys_valid = true;

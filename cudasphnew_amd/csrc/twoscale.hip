@@ -1,14 +1,665 @@
-// twoscale.hip -- two-scale sub-particle pass (AdvanceTwoScale,
-// particle_simulation.cu:2145-2182).
+// twoscale.hip -- the two-scale sub-particle pass of AdvanceTwoScale
+// (particle_simulation.cu:2145-2182), device-resident.
+//
+// Runs between the coarse density pass and the coarse force pass (the sub
+// pass reads coarse positions, velocities and densities before integration,
+// as the reference's order does; nothing feeds back into the coarse step):
+//   state    shift_state + find_split_particles + find_boundary_particles
+//            (cu:730-740, 932-1017); the boundary bit is pulled from split
+//            neighbours instead of the reference's racy byte OR.
+//   collect  check_split_boundary_default + 3 exclusive scans + collect_ids
+//            (cu:1092-1162, 2509-2567) as one 3-way compaction over
+//            original ids; counts stay on the device (the reference copies
+//            them to the host every step, cu:2532-2540).
+//   seed     initialize_sub_particles (cu:1023-1090) + project_quantities
+//            (cu:570-622) for the boundary sub-particles.
+//   sort     sub hash + sort + cell ranges on the sub grid (cu:121-153,
+//            2327-2389); the radix sort is stable over the sub list, whose
+//            order (split parents then boundary parents, by id, children in
+//            order) is the reference's tie order.
+//   physics  sub density (split subs), sub acceleration (split subs), sub
+//            integration + collision (all subs) (cu:259-313, 477-568,
+//            667-696, 836-930).
+// Every kernel reads the live counts from device memory; grids are sized
+// for the sub-particle capacity.
+#include <cstring>
+#include <vector>
+
 #include "engine.h"
 
 namespace sph {
 
-sph_status twoscale_create(Engine& e) { return e.fail(SPH_INVALID_ARG, "two-scale pass not available"); }
-void twoscale_destroy(Engine& e) { e.ts = nullptr; }
-void twoscale_enqueue(Engine&) {}
-sph_status twoscale_read(Engine& e, sph_field, void*, size_t) { return e.fail(SPH_INVALID_ARG, "no two-scale data"); }
-sph_status twoscale_counts(Engine&, sph_counts*) { return SPH_OK; }
-size_t twoscale_bytes(const Engine&) { return 0; }
+struct TSCounts {
+    uint32_t nsplit, nboundary, ndefault, nsub, nparents, nsub_prev, pad0, pad1;
+};
+
+struct TwoScale {
+    uint32_t cap = 0;            // sub-particle capacity
+    uint32_t nblk = 0;           // compaction blocks
+    uint32_t sub_cells = 0;
+    int sub_bits = 1;
+    uint8_t* state_fin = nullptr;   // final state, sorted order
+    uint8_t* cls = nullptr;         // class by original id: 0 default, 1 split, 2 boundary
+    uint32_t *split_ids = nullptr, *boundary_ids = nullptr, *default_ids = nullptr;
+    uint32_t* blk = nullptr;        // [3][nblk] block counts -> offsets
+    TSCounts* counts = nullptr;
+    float4 *spos = nullptr, *svel = nullptr;          // list order (seeded)
+    uint32_t *sid = nullptr, *skey = nullptr;         // list order
+    uint32_t *sskey = nullptr, *sperm = nullptr;      // sorted order
+    float4 *spos_s = nullptr, *svel_s = nullptr;      // sorted order
+    float4 *spos_out = nullptr, *svel_out = nullptr, *sacc = nullptr;  // sorted, after the step
+    int2* srange = nullptr;
+    RadixSort sorter;
+    size_t bytes = 0;
+    bool sub_cells_valid = false;
+};
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kCompactItems = 4;
+constexpr int kCompactTile = kB * kCompactItems;
+
+inline unsigned nblocks(uint32_t n) { return (n + kB - 1) / kB; }
+
+// shift_state + find_split_particles, sorted order (own slot only).
+__global__ __launch_bounds__(kB) void k_ts_split(const float4* __restrict__ pos, uint8_t* __restrict__ state,
+                                                 uint32_t n, const DevScene* __restrict__ scenes) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[0];
+    float4 x = pos[s];
+    uint8_t st = (uint8_t)(state[s] << 2);
+    bool in = x.x >= S.split_lo[0] && x.x <= S.split_hi[0] && x.y >= S.split_lo[1] && x.y <= S.split_hi[1] &&
+              x.z >= S.split_lo[2] && x.z <= S.split_hi[2];
+    if (in) st |= 1;
+    state[s] = st;
+}
+
+// find_boundary_particles as a pull: bit 2 if any particle within h has its
+// split bit set this step.  Also records the class by original id.
+__global__ __launch_bounds__(kB) void k_ts_boundary(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
+                                                    const int2* __restrict__ range, const uint32_t* __restrict__ oid,
+                                                    uint32_t n, const DevScene* __restrict__ scenes,
+                                                    uint8_t* __restrict__ state_fin, uint8_t* __restrict__ cls) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[0];
+    float4 xi = pos[s];
+    const float h = S.h, h2 = S.h2;
+    int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
+    int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
+    int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+    bool hit = false;
+    for (int k = lz; k <= hz && !hit; ++k)
+        for (int j = ly; j <= hy && !hit; ++j)
+            for (int i = lx; i <= hx && !hit; ++i) {
+                int2 r = range[S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j) + i)];
+                for (int q = r.x; q < r.y; ++q) {
+                    if (!(state[q] & 1)) continue;
+                    float4 xj = pos[q];
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    if (dx * dx + dy * dy + dz * dz <= h2) { hit = true; break; }
+                }
+            }
+    uint8_t st = state[s];
+    if (hit) st |= 2;
+    state_fin[s] = st;
+    int c = st & 3;
+    cls[oid[s]] = c == 0 ? 0 : (c == 2 ? 2 : 1);
+}
+
+// 3-way compaction in original-id order: per-tile counts ...
+__global__ __launch_bounds__(kB) void k_ts_count(const uint8_t* __restrict__ cls, uint32_t n,
+                                                 uint32_t* __restrict__ blk, uint32_t nblk) {
+    __shared__ uint32_t s_c[3];
+    if (threadIdx.x < 3) s_c[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t c[3] = {0, 0, 0};
+    uint32_t base = blockIdx.x * kCompactTile;
+    for (int r = 0; r < kCompactItems; ++r) {
+        uint32_t i = base + r * kB + threadIdx.x;
+        if (i < n) c[cls[i]]++;
+    }
+    for (int t = 0; t < 3; ++t) {
+        uint32_t v = c[t];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if ((threadIdx.x & 63) == 0 && v) atomicAdd(&s_c[t], v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) blk[threadIdx.x * nblk + blockIdx.x] = s_c[threadIdx.x];
+}
+
+// ... one block scans the tile counts, writes the totals and the capacity
+// check ...
+__global__ __launch_bounds__(kB) void k_ts_scan(uint32_t* __restrict__ blk, uint32_t nblk, TSCounts* __restrict__ cnt,
+                                                uint32_t cap, uint32_t* __restrict__ status) {
+    __shared__ uint32_t s_w[kB / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t tot[3];
+    for (int t = 0; t < 3; ++t) {
+        uint32_t carry = 0;
+        uint32_t* row = blk + t * nblk;
+        for (uint32_t b0 = 0; b0 < nblk; b0 += kB) {
+            uint32_t b = b0 + threadIdx.x;
+            uint32_t v = b < nblk ? row[b] : 0u;
+            uint32_t x = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (lane == 63) s_w[wave] = x;
+            __syncthreads();
+            uint32_t off = 0, all = 0;
+            for (int w = 0; w < kB / 64; ++w) {
+                if (w < wave) off += s_w[w];
+                all += s_w[w];
+            }
+            __syncthreads();
+            if (b < nblk) row[b] = carry + off + x - v;
+            carry += all;
+        }
+        tot[t] = carry;
+    }
+    if (threadIdx.x == 0) {
+        uint32_t nsplit = tot[1], nbound = tot[2];
+        uint64_t nsub = 8ull * (nsplit + nbound);
+        if (nsub > cap) {
+            atomicOr(status, kStatusCapacity);
+            // keep whole parents that fit, split parents first
+            uint32_t fit = cap / 8;
+            if (nsplit > fit) { nsplit = fit; nbound = 0; }
+            else if (nsplit + nbound > fit) nbound = fit - nsplit;
+            nsub = 8ull * (nsplit + nbound);
+        }
+        cnt->nsub_prev = cnt->nsub;
+        cnt->nsplit = nsplit;
+        cnt->nboundary = nbound;
+        cnt->ndefault = tot[0];
+        cnt->nparents = nsplit + nbound;
+        cnt->nsub = (uint32_t)nsub;
+    }
+}
+
+// ... and every element writes its id into its list (collect_ids).
+__global__ __launch_bounds__(kB) void k_ts_collect(const uint8_t* __restrict__ cls, uint32_t n,
+                                                   const uint32_t* __restrict__ blk, uint32_t nblk,
+                                                   const TSCounts* __restrict__ cnt, uint32_t* __restrict__ split_ids,
+                                                   uint32_t* __restrict__ boundary_ids,
+                                                   uint32_t* __restrict__ default_ids) {
+    __shared__ uint32_t s_w[3][kB / 64];
+    __shared__ uint32_t s_carry[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x < 3) s_carry[threadIdx.x] = blk[threadIdx.x * nblk + blockIdx.x];
+    __syncthreads();
+    const uint32_t nsplit = cnt->nsplit, nbound = cnt->nboundary;
+    uint32_t base = blockIdx.x * kCompactTile;
+    for (int r = 0; r < kCompactItems; ++r) {
+        uint32_t i = base + r * kB + threadIdx.x;
+        int c = i < n ? cls[i] : -1;
+        uint32_t rank[3], tot[3];
+        for (int t = 0; t < 3; ++t) {
+            uint64_t m = __ballot(c == t);
+            uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+            rank[t] = (uint32_t)__popcll(m & below);
+            if (lane == 0) s_w[t][wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        for (int t = 0; t < 3; ++t) {
+            uint32_t off = 0, all = 0;
+            for (int w = 0; w < kB / 64; ++w) {
+                if (w < wave) off += s_w[t][w];
+                all += s_w[t][w];
+            }
+            rank[t] += off + s_carry[t];
+            tot[t] = all;
+        }
+        if (c == 1 && rank[1] < nsplit) split_ids[rank[1]] = i;
+        if (c == 2 && rank[2] < nbound) boundary_ids[rank[2]] = i;
+        if (c == 0) default_ids[rank[0]] = i;
+        __syncthreads();
+        if (threadIdx.x < 3) s_carry[threadIdx.x] += tot[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// initialize_sub_particles for every split and boundary parent (cu:1023-
+// 1090): 8 children, directions (+-1,+-1,+-1) in the order of the reference
+// table (x sign from bit 2, y from bit 1, z from bit 0 of the child index).
+__global__ __launch_bounds__(kB) void k_ts_seed(const TSCounts* __restrict__ cnt,
+                                                const uint32_t* __restrict__ split_ids,
+                                                const uint32_t* __restrict__ boundary_ids,
+                                                const uint32_t* __restrict__ slot_of_oid,
+                                                const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                const DevScene* __restrict__ scenes, float4* __restrict__ spos,
+                                                float4* __restrict__ svel, uint32_t* __restrict__ sid) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nsub = cnt->nsub, nsplit = cnt->nsplit;
+    if (k >= nsub) return;
+    const DevScene& S = scenes[0];
+    uint32_t q = k >> 3, c = k & 7;
+    uint32_t id = q < nsplit ? split_ids[q] : boundary_ids[q - nsplit];
+    uint32_t slot = slot_of_oid[id];
+    float4 xp = pos[slot];
+    float4 vp = vel[slot];
+    float radius, dirf;
+    if (S.sub_placement == 0) {
+        // r = (3 m / (4 pi rho))^(1/3), along the cube diagonals
+        float radicand = 3.0f * S.mass / (4.0f * 3.14159265358979323846f * vp.w);
+        radius = cbrtf(radicand);
+        dirf = 0.577350269f;   // 1/sqrt(3) (cu:1025)
+    } else {
+        radius = 0.25f * S.h;
+        dirf = 1.0f;
+    }
+    float off = radius * dirf;
+    float ox = (c & 4) ? -off : off, oy = (c & 2) ? -off : off, oz = (c & 1) ? -off : off;
+    spos[k] = make_float4(xp.x + ox, xp.y + oy, xp.z + oz, 0.0f);
+    svel[k] = make_float4(vp.x, vp.y, vp.z, 0.0f);
+    sid[k] = 8u * id + c;
+}
+
+// project_quantities (cu:570-622, 1328-1381): boundary sub-particles take the
+// unweighted mean density of the coarse particles within h_sub, searched on
+// the coarse grid; the mean pressure is k (mean density - rho0) since the
+// equation of state is linear.
+__global__ __launch_bounds__(kB) void k_ts_project(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
+                                                   float4* __restrict__ svel, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const int2* __restrict__ range,
+                                                   const DevScene* __restrict__ scenes) {
+    const uint32_t nsub = cnt->nsub, first = 8u * cnt->nsplit;
+    uint32_t k = first + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nsub) return;
+    const DevScene& S = scenes[0];
+    float4 xs = spos[k];
+    const float hs = S.h_sub, hs2 = S.h2_sub;
+    int lx = range_coord(xs.x, -hs, S.ox, S.cell, S.nx), hx = range_coord(xs.x, hs, S.ox, S.cell, S.nx);
+    int ly = range_coord(xs.y, -hs, S.oy, S.cell, S.ny), hy = range_coord(xs.y, hs, S.oy, S.cell, S.ny);
+    int lz = range_coord(xs.z, -hs, S.oz, S.cell, S.nz), hz = range_coord(xs.z, hs, S.oz, S.cell, S.nz);
+    float sum = 0.0f, cntn = 0.0f;
+    for (int kk = lz; kk <= hz; ++kk)
+        for (int j = ly; j <= hy; ++j)
+            for (int i = lx; i <= hx; ++i) {
+                int2 r = range[S.cell_offset + (uint32_t)(S.nx * (S.ny * kk + j) + i)];
+                for (int q = r.x; q < r.y; ++q) {
+                    float4 xj = pos[q];
+                    float dx = xs.x - xj.x, dy = xs.y - xj.y, dz = xs.z - xj.z;
+                    if (dx * dx + dy * dy + dz * dz <= hs2) {
+                        sum += vel[q].w;
+                        cntn += 1.0f;
+                    }
+                }
+            }
+    svel[k].w = cntn > 0.0f ? sum / cntn : S.rho0;
+}
+
+__global__ __launch_bounds__(kB) void k_ts_hash(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
+                                                uint32_t* __restrict__ skey, const DevScene* __restrict__ scenes,
+                                                uint32_t* __restrict__ status) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt->nsub) return;
+    float4 x = spos[k];
+    uint32_t flags = 0;
+    skey[k] = sub_cell_key(scenes[0], x.x, x.y, x.z, flags);
+    if (flags) atomicOr(status, flags);
+}
+
+__global__ __launch_bounds__(kB) void k_ts_cells_clear(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sskey,
+                                                       int2* __restrict__ srange) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->nsub_prev) return;
+    uint32_t k = sskey[t];
+    if (t == 0 || sskey[t - 1] != k) srange[k] = make_int2(-1, -1);
+}
+
+// sub cell ranges + gather into sub-sorted order (stable list order = the
+// reference's tie order, no fix-up needed).
+__global__ __launch_bounds__(kB) void k_ts_cells_gather(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sskey,
+                                                        const uint32_t* __restrict__ sperm,
+                                                        const float4* __restrict__ spos, const float4* __restrict__ svel,
+                                                        int2* __restrict__ srange, float4* __restrict__ spos_s,
+                                                        float4* __restrict__ svel_s) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = cnt->nsub;
+    if (t >= n) return;
+    uint32_t k = sskey[t];
+    int* r = reinterpret_cast<int*>(srange);
+    if (t == 0 || sskey[t - 1] != k) r[2 * (size_t)k] = (int)t;
+    if (t == n - 1 || sskey[t + 1] != k) r[2 * (size_t)k + 1] = (int)(t + 1);
+    uint32_t src = sperm[t];
+    spos_s[t] = spos[src];
+    svel_s[t] = svel[src];
+}
+
+__device__ __forceinline__ void sub_range(const DevScene& S, float4 x, float h, int& lx, int& hx, int& ly, int& hy,
+                                          int& lz, int& hz) {
+    lx = range_coord(x.x, -h, S.ox, S.cell_sub, S.snx); hx = range_coord(x.x, h, S.ox, S.cell_sub, S.snx);
+    ly = range_coord(x.y, -h, S.oy, S.cell_sub, S.sny); hy = range_coord(x.y, h, S.oy, S.cell_sub, S.sny);
+    lz = range_coord(x.z, -h, S.oz, S.cell_sub, S.snz); hz = range_coord(x.z, h, S.oz, S.cell_sub, S.snz);
+}
+
+// compute_sub_particle_density_pressure (cu:259-313): split subs only, no
+// wall term.
+__global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
+                                                   const float4* __restrict__ spos_s, float4* __restrict__ svel_s,
+                                                   const int2* __restrict__ srange,
+                                                   const DevScene* __restrict__ scenes) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->nsub) return;
+    if (sperm[t] >= 8u * cnt->nsplit) return;   // boundary sub: projected density
+    const DevScene& S = scenes[0];
+    float4 xi = spos_s[t];
+    const float h2 = S.h2_sub;
+    int lx, hx, ly, hy, lz, hz;
+    sub_range(S, xi, S.h_sub, lx, hx, ly, hy, lz, hz);
+    float sum = 0.0f;
+    for (int k = lz; k <= hz; ++k)
+        for (int j = ly; j <= hy; ++j) {
+            uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
+            int b = 0x7fffffff, e = -1;
+            for (int i = lx; i <= hx; ++i) {
+                int2 r = srange[row + (uint32_t)i];
+                if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
+            }
+            for (int q = b; q < e; ++q) {
+                float4 xj = spos_s[q];
+                float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
+                sum += d * d * d;
+            }
+        }
+    reinterpret_cast<float*>(svel_s)[4 * (size_t)t + 3] = sum * (S.poly6_sub * S.mass_sub);
+}
+
+// compute_sub_particle_acceleration (cu:477-568) for split subs (boundary
+// subs: zero acceleration), then integrate_sub_particles_euler (cu:667-696)
+// and the collision response for every sub-particle.
+__global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
+                                                 const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
+                                                 const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
+                                                 const float* __restrict__ gridd, float4* __restrict__ spos_out,
+                                                 float4* __restrict__ svel_out, float4* __restrict__ sacc,
+                                                 uint32_t* __restrict__ status) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->nsub) return;
+    const DevScene& S = scenes[0];
+    const float4 xi = spos_s[t];
+    const float4 vi = svel_s[t];
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    if (sperm[t] < 8u * cnt->nsplit) {
+        const float h = S.h_sub, h2 = S.h2_sub;
+        const float rhoi = vi.w;
+        const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;
+        const float kr0 = S.stiff * S.rho0;
+        int lx, hx, ly, hy, lz, hz;
+        sub_range(S, xi, h, lx, hx, ly, hy, lz, hz);
+        float fpx = 0, fpy = 0, fpz = 0, fvx = 0, fvy = 0, fvz = 0, cgx = 0, cgy = 0, cgz = 0, cl = 0;
+        for (int k = lz; k <= hz; ++k)
+            for (int j = ly; j <= hy; ++j) {
+                uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
+                int b = 0x7fffffff, e = -1;
+                for (int i = lx; i <= hx; ++i) {
+                    int2 r = srange[row + (uint32_t)i];
+                    if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
+                }
+                for (int q = b; q < e; ++q) {
+                    float4 xj = spos_s[q];
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 <= h2 && r2 > 0.0f) {
+                        float4 vj = svel_s[q];
+                        float inv_r = __builtin_amdgcn_rsqf(r2);
+                        float rn = r2 * inv_r;
+                        float invj = __builtin_amdgcn_rcpf(vj.w);
+                        float pj = S.stiff * vj.w - kr0;
+                        float hr = h - rn, qd = h2 - r2;
+                        float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
+                        fpx += cp * dx; fpy += cp * dy; fpz += cp * dz;
+                        float wv = invj * hr;
+                        fvx += wv * (vj.x - vi.x); fvy += wv * (vj.y - vi.y); fvz += wv * (vj.z - vi.z);
+                        float wc = invj * qd * qd;
+                        cgx += wc * dx; cgy += wc * dy; cgz += wc * dz;
+                        cl += invj * qd * (3.0f * h2 - 7.0f * r2);
+                    }
+                }
+            }
+        const float cpf = -S.mass_sub * S.grad_spiky_sub, cvf = S.mu * S.mass_sub * S.lapl_visc_sub;
+        float Fx = cvf * fvx + cpf * fpx, Fy = cvf * fvy + cpf * fpy, Fz = cvf * fvz + cpf * fpz;
+        float cm = S.mass_sub * S.grad_poly6_sub;
+        float gx = cm * cgx, gy = cm * cgy, gz = cm * cgz;
+        float gnorm = sqrtf(gx * gx + gy * gy + gz * gz);
+        if (gnorm > S.norm_thresh) {
+            float fc = S.sigma * (S.mass_sub * S.lapl_poly6_sub * cl) / gnorm;
+            Fx -= fc * gx; Fy -= fc * gy; Fz -= fc * gz;
+        }
+        ax = Fx / rhoi;
+        ay = Fy / rhoi - S.gravity;
+        az = Fz / rhoi;
+    }
+    sacc[t] = make_float4(ax, ay, az, 0.0f);
+    const float dt = S.dt_sub;
+    float vx = vi.x + dt * ax, vy = vi.y + dt * ay, vz = vi.z + dt * az;
+    float px = xi.x + dt * vx, py = xi.y + dt * vy, pz = xi.z + dt * vz;
+    if (S.collision) {
+        float cx, cy, cz;
+        texel_coords(S, px, py, pz, cx, cy, cz);
+        float D = dist_at(gridd, S, cx, cy, cz);
+        if (D > 0.0f) {
+            float3 gw = dist_grad_normalized(gridd, S, cx, cy, cz);
+            gw.x *= S.bscale_x * (1.0f / (float)S.bnx);
+            gw.y *= S.bscale_y * (1.0f / (float)S.bny);
+            gw.z *= S.bscale_z * (1.0f / (float)S.bnz);
+            float gl = sqrtf(gw.x * gw.x + gw.y * gw.y + gw.z * gw.z);
+            if (gl > 0.0f) {
+                float nx_ = gw.x / gl, ny_ = gw.y / gl, nz_ = gw.z / gl;
+                float vn = sqrtf(vx * vx + vy * vy + vz * vz);
+                float dp = nx_ * vx + ny_ * vy + nz_ * vz;
+                float c = 1.0f;
+                if (S.restitution != 0.0f && vn > 0.0f) c += S.restitution * D / (dt * vn);
+                c *= dp;
+                vx -= c * nx_; vy -= c * ny_; vz -= c * nz_;
+                px -= D * nx_; py -= D * ny_; pz -= D * nz_;
+            }
+        }
+    }
+    if (!(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(vx) && isfinite(vy) && isfinite(vz)))
+        atomicOr(status, kStatusNonFinite);
+    spos_out[t] = make_float4(px, py, pz, 0.0f);
+    svel_out[t] = make_float4(vx, vy, vz, vi.w);
+}
+
+__global__ void k_ts_prev(TSCounts* __restrict__ cnt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) cnt->nsub_prev = cnt->nsub;
+}
+
+}  // namespace
+
+template <class T>
+static hipError_t ts_alloc(TwoScale& t, T** p, size_t n) {
+    size_t b = sizeof(T) * (n ? n : 1);
+    t.bytes += b;
+    return hipMalloc(reinterpret_cast<void**>(p), b);
+}
+
+sph_status twoscale_create(Engine& e) {
+    if (e.nscenes != 1) return e.fail(SPH_INVALID_ARG, "two-scale pass runs on single-scene handles");
+    const sph_params& P = e.params[0];
+    for (int a = 0; a < 3; ++a)
+        if (P.grid_dim_sub[a] < 1) return e.fail(SPH_INVALID_ARG, "sub grid dims must be >= 1");
+    if (!(P.grid_spacing_sub > 0.0f) || !(P.compact_support_sub > 0.0f) || !(P.sub_particle_mass > 0.0f))
+        return e.fail(SPH_INVALID_ARG, "sub-particle parameters must be positive");
+    double sc = (double)P.grid_dim_sub[0] * P.grid_dim_sub[1] * P.grid_dim_sub[2];
+    if (sc > 2147483647.0) return e.fail(SPH_INVALID_ARG, "sub grid too large");
+    TwoScale* t = new TwoScale();
+    uint64_t cap = P.max_sub_particles ? P.max_sub_particles : 8ull * e.N;
+    cap = (cap / 8) * 8;
+    if (cap < 8) cap = 8;
+    if (cap > 0x7ffffff8ull) cap = 0x7ffffff8ull;
+    t->cap = (uint32_t)cap;
+    t->nblk = (e.N + kCompactTile - 1) / kCompactTile;
+    t->sub_cells = (uint32_t)sc;
+    t->sub_bits = 1;
+    while (t->sub_bits < 32 && (1ull << t->sub_bits) < (uint64_t)t->sub_cells) ++t->sub_bits;
+    bool ok = ts_alloc(*t, &t->state_fin, e.N) == hipSuccess && ts_alloc(*t, &t->cls, e.N) == hipSuccess &&
+              ts_alloc(*t, &t->split_ids, e.N) == hipSuccess && ts_alloc(*t, &t->boundary_ids, e.N) == hipSuccess &&
+              ts_alloc(*t, &t->default_ids, e.N) == hipSuccess &&
+              ts_alloc(*t, &t->blk, 3 * (size_t)t->nblk) == hipSuccess && ts_alloc(*t, &t->counts, 1) == hipSuccess &&
+              ts_alloc(*t, &t->spos, t->cap) == hipSuccess && ts_alloc(*t, &t->svel, t->cap) == hipSuccess &&
+              ts_alloc(*t, &t->sid, t->cap) == hipSuccess && ts_alloc(*t, &t->skey, t->cap) == hipSuccess &&
+              ts_alloc(*t, &t->sskey, t->cap) == hipSuccess && ts_alloc(*t, &t->sperm, t->cap) == hipSuccess &&
+              ts_alloc(*t, &t->spos_s, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_s, t->cap) == hipSuccess &&
+              ts_alloc(*t, &t->spos_out, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_out, t->cap) == hipSuccess &&
+              ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess &&
+              t->sorter.init(t->cap) == hipSuccess;
+    e.ts = t;   // owned by the engine from here on (freed by twoscale_destroy)
+    if (!ok) {
+        (void)hipGetLastError();
+        return e.fail(SPH_OUT_OF_MEMORY, "two-scale buffers");
+    }
+    t->bytes += t->sorter.bytes();
+    if (hipMemset(t->srange, 0xFF, (size_t)t->sub_cells * sizeof(int2)) != hipSuccess ||
+        hipMemset(t->counts, 0, sizeof(TSCounts)) != hipSuccess ||
+        hipMemset(t->state_fin, 0, e.N) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return e.fail(SPH_DEVICE_ERROR, "two-scale initial clear");
+    return SPH_OK;
+}
+
+void twoscale_destroy(Engine& e) {
+    TwoScale* t = e.ts;
+    if (!t) return;
+    void* ptrs[] = {t->state_fin, t->cls,   t->split_ids, t->boundary_ids, t->default_ids, t->blk,
+                    t->counts,    t->spos,  t->svel,      t->sid,          t->skey,        t->sskey,
+                    t->sperm,     t->spos_s, t->svel_s,   t->spos_out,     t->svel_out,    t->sacc,
+                    t->srange};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    t->sorter.release();
+    delete t;
+    e.ts = nullptr;
+}
+
+void twoscale_enqueue(Engine& e) {
+    TwoScale& t = *e.ts;
+    hipStream_t st = e.stream;
+    const uint32_t n = e.N;
+    hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, n, e.d_scenes);
+    hipLaunchKernelGGL(k_ts_boundary, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, e.range, e.oid_b, n,
+                       e.d_scenes, t.state_fin, t.cls);
+    e.mark(kStageTwoScaleState);
+    hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
+    if (t.sub_cells_valid)
+        hipLaunchKernelGGL(k_ts_cells_clear, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.srange);
+    hipLaunchKernelGGL(k_ts_scan, dim3(1), dim3(kB), 0, st, t.blk, t.nblk, t.counts, t.cap, e.status);
+    hipLaunchKernelGGL(k_ts_collect, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk, t.counts, t.split_ids,
+                       t.boundary_ids, t.default_ids);
+    e.mark(kStageTwoScaleCollect);
+    hipLaunchKernelGGL(k_ts_seed, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.split_ids, t.boundary_ids,
+                       e.slot_of_oid, e.pos_b, e.vel_b, e.d_scenes, t.spos, t.svel, t.sid);
+    hipLaunchKernelGGL(k_ts_project, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_b,
+                       e.vel_b, e.range, e.d_scenes);
+    e.mark(kStageTwoScaleSeed);
+    hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
+                       e.status);
+    t.sorter.sort(st, t.skey, nullptr, t.sskey, t.sperm, &t.counts->nsub, t.sub_bits);
+    hipLaunchKernelGGL(k_ts_cells_gather, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm, t.spos,
+                       t.svel, t.srange, t.spos_s, t.svel_s);
+    t.sub_cells_valid = true;
+    e.mark(kStageTwoScaleSubSort);
+    hipLaunchKernelGGL(k_ts_density, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
+                       t.srange, e.d_scenes);
+    hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
+                       t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
+    hipLaunchKernelGGL(k_ts_prev, dim3(1), dim3(64), 0, st, t.counts);
+    e.mark(kStageTwoScaleSubPhysics);
+}
+
+sph_status twoscale_counts(Engine& e, sph_counts* out) {
+    TSCounts c;
+    if (hipStreamSynchronize(e.stream) != hipSuccess ||
+        hipMemcpy(&c, e.ts->counts, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+        return e.fail(SPH_DEVICE_ERROR, "count read failed");
+    out->num_split = c.nsplit;
+    out->num_boundary = c.nboundary;
+    out->num_default = c.ndefault;
+    out->num_sub = c.nsub;
+    return SPH_OK;
+}
+
+size_t twoscale_bytes(const Engine& e) { return e.ts ? e.ts->bytes : 0; }
+
+uint8_t* twoscale_final_state(Engine& e) { return e.ts->state_fin; }
+
+sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t nbytes) {
+    TwoScale& t = *e.ts;
+    TSCounts c;
+    if (hipMemcpy(&c, t.counts, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+        return e.fail(SPH_DEVICE_ERROR, "count read failed");
+    const uint32_t ns = c.nsub;
+    auto need = [&](size_t want) -> bool { return nbytes == want; };
+    auto cp = [&](void* h, const void* d, size_t b) -> sph_status {
+        if (b && hipMemcpy(h, d, b, hipMemcpyDeviceToHost) != hipSuccess)
+            return e.fail(SPH_DEVICE_ERROR, "device-to-host copy failed");
+        return SPH_OK;
+    };
+    sph_status st;
+    std::vector<uint32_t> perm(ns);
+    auto load_perm = [&]() { return cp(perm.data(), t.sperm, ns * sizeof(uint32_t)); };
+    switch (f) {
+    case SPH_FIELD_SUB_POSITION:
+    case SPH_FIELD_SUB_VELOCITY:
+    case SPH_FIELD_SUB_ACCEL: {
+        if (!need(3 * (size_t)ns * sizeof(float))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        const float4* src = f == SPH_FIELD_SUB_POSITION ? t.spos_out : (f == SPH_FIELD_SUB_VELOCITY ? t.svel_out : t.sacc);
+        std::vector<float4> tmp(ns);
+        if ((st = cp(tmp.data(), src, ns * sizeof(float4))) != SPH_OK || (st = load_perm()) != SPH_OK) return st;
+        float* o = (float*)dst;
+        for (uint32_t i = 0; i < ns; ++i) {
+            uint32_t k = perm[i];
+            o[3 * (size_t)k] = tmp[i].x;
+            o[3 * (size_t)k + 1] = tmp[i].y;
+            o[3 * (size_t)k + 2] = tmp[i].z;
+        }
+        return SPH_OK;
+    }
+    case SPH_FIELD_SUB_DENSITY:
+    case SPH_FIELD_SUB_PRESSURE: {
+        if (!need((size_t)ns * sizeof(float))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        std::vector<float4> tmp(ns);
+        if ((st = cp(tmp.data(), t.svel_s, ns * sizeof(float4))) != SPH_OK || (st = load_perm()) != SPH_OK) return st;
+        float* o = (float*)dst;
+        const sph_params& P = e.params[0];
+        for (uint32_t i = 0; i < ns; ++i) {
+            float rho = tmp[i].w;
+            o[perm[i]] = f == SPH_FIELD_SUB_DENSITY ? rho : P.gas_stiffness * (rho - P.rest_density);
+        }
+        return SPH_OK;
+    }
+    case SPH_FIELD_SUB_IDS:
+        if (!need((size_t)ns * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        return cp(dst, t.sid, ns * sizeof(uint32_t));
+    case SPH_FIELD_SUB_SORTED_KEYS:
+        if (!need((size_t)ns * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        return cp(dst, t.sskey, ns * sizeof(uint32_t));
+    case SPH_FIELD_SUB_SORTED_IDS: {
+        if (!need((size_t)ns * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        std::vector<uint32_t> sid(ns);
+        if ((st = cp(sid.data(), t.sid, ns * sizeof(uint32_t))) != SPH_OK || (st = load_perm()) != SPH_OK) return st;
+        uint32_t* o = (uint32_t*)dst;
+        for (uint32_t i = 0; i < ns; ++i) o[i] = sid[perm[i]];
+        return SPH_OK;
+    }
+    case SPH_FIELD_SPLIT_IDS:
+        if (!need((size_t)c.nsplit * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        return cp(dst, t.split_ids, c.nsplit * sizeof(uint32_t));
+    case SPH_FIELD_BOUNDARY_IDS:
+        if (!need((size_t)c.nboundary * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        return cp(dst, t.boundary_ids, c.nboundary * sizeof(uint32_t));
+    case SPH_FIELD_DEFAULT_IDS:
+        if (!need((size_t)c.ndefault * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        return cp(dst, t.default_ids, c.ndefault * sizeof(uint32_t));
+    default:
+        return e.fail(SPH_INVALID_ARG, "field not available");
+    }
+}
 
 }  // namespace sph

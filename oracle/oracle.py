@@ -19,6 +19,7 @@ FLAG_OUT_OF_DOMAIN = 1
 FLAG_NONFINITE = 2
 FIELD_COLOR_GRAD_NORM = 100
 FIELD_WALL_DISTANCE = 101
+FIELD_SUB_COLOR_GRAD_NORM = 102
 
 
 def _load():
@@ -125,6 +126,15 @@ class OracleSim:
         # exactly) is not ambiguous: both precisions skip the wall term
         wall = np.where(dw == r0, np.inf, np.abs(dw / r0 - 1.0))
         return np.minimum(np.abs(cg / thr - 1.0), wall)
+
+    def sub_branch_margin(self) -> np.ndarray:
+        """Relative distance of each sub-particle's colour-gradient norm from
+        normThresh (inf for sub-particles without an acceleration pass)."""
+        n = self.counts().num_sub
+        a = np.empty(n, np.float64)
+        if lib.orc_read(self._s, FIELD_SUB_COLOR_GRAD_NORM, a.ctypes.data_as(C.c_void_p), a.nbytes) < 0:
+            raise RuntimeError("orc_read failed")
+        return np.abs(a / float(self._p.norm_thresh) - 1.0)
 
     def read(self, field) -> np.ndarray:
         field = Field(field)

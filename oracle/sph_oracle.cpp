@@ -203,7 +203,7 @@ struct Sim {
     // branch variables of the acceleration (colour-gradient norm against
     // normThresh, wall distance against r0), for tests that must skip
     // particles sitting on a discontinuity of the reference's model
-    std::vector<double> cgn_out, dwall_out;
+    std::vector<double> cgn_out, dwall_out, sub_cgn_out;
     std::vector<uint8_t> state;              // N
     std::vector<int32_t> keys, skeys, sids, cstart, cend;
 
@@ -458,6 +458,7 @@ struct Sim {
         sx.assign(3 * (size_t)nsub, 0);
         sv.assign(3 * (size_t)nsub, 0);
         sa.assign(3 * (size_t)nsub, 0);
+        sub_cgn_out.assign(nsub, 1e300);
         srho.assign(nsub, 0);
         sp.assign(nsub, 0);
         for (uint32_t q = 0; q < nsub / 8; ++q) {
@@ -593,6 +594,7 @@ struct Sim {
                 }
             });
             R cgn = std::sqrt(cg[0] * cg[0] + cg[1] * cg[1] + cg[2] * cg[2]);
+            sub_cgn_out[k] = (double)cgn;
             if (cgn > (R)P.norm_thresh) {
                 R fc = (R)P.tension_coefficient * cl / cgn;
                 for (int c = 0; c < 3; ++c) F[c] -= fc * cg[c];
@@ -683,6 +685,7 @@ struct Sim {
         case SPH_FIELD_SUB_SORTED_KEYS: return put_int(sub_skeys.data(), nsub);
         case ORC_FIELD_COLOR_GRAD_NORM: return put_dbl(cgn_out, N);
         case ORC_FIELD_WALL_DISTANCE: return put_dbl(dwall_out, N);
+        case ORC_FIELD_SUB_COLOR_GRAD_NORM: return put_dbl(sub_cgn_out, nsub);
         case SPH_FIELD_SUB_SORTED_IDS: {
             if (bytes < nsub * sizeof(int32_t)) return -1;
             int32_t* o = (int32_t*)dst;

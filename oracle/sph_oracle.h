@@ -31,6 +31,8 @@ extern "C" {
  * compared with r0 (the two switches of particle_simulation.cu:407,457) */
 #define ORC_FIELD_COLOR_GRAD_NORM 100
 #define ORC_FIELD_WALL_DISTANCE 101
+/* colour-gradient norm of each sub-particle (sub-list order) */
+#define ORC_FIELD_SUB_COLOR_GRAD_NORM 102
 
 typedef struct orc_sim orc_sim;
 

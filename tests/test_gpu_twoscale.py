@@ -1,0 +1,133 @@
+"""Two-scale pass (AdvanceTwoScale) on the GPU against the oracle.
+
+Classification, lists and sub ids are integer results: bit-exact.  Sub
+densities within 1e-5 relative, sub accelerations within 1e-4 of
+max(|a|, g) away from the surface-tension switch, like the coarse step.
+"""
+import numpy as np
+import pytest
+
+import cudasphnew_amd as S
+from cudasphnew_amd import Field
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+G = 9.81
+THREADS = 16
+
+
+def _check_sub_accel(sim, orc, tol=1e-4, margin=2e-5):
+    a = sim.read(Field.SUB_ACCEL)
+    a_o = orc.read(Field.SUB_ACCEL)
+    scale = np.maximum(np.linalg.norm(a_o, axis=1), G)[:, None]
+    err = (np.abs(a - a_o) / scale).max(1)
+    on = orc.sub_branch_margin() < margin
+    assert on.sum() <= max(2, 1e-3 * a.shape[0])
+    if (~on).any():
+        assert err[~on].max() <= tol, err[~on].max()
+
+
+def _compare_two_scale(sim, orc, n_coarse):
+    c, co = sim.counts(), orc.counts()
+    assert (c.num_split, c.num_boundary, c.num_default, c.num_sub) == \
+        (co.num_split, co.num_boundary, co.num_default, co.num_sub)
+    for f in (Field.SPLIT_IDS, Field.BOUNDARY_IDS, Field.DEFAULT_IDS, Field.SUB_IDS, Field.STATE):
+        assert np.array_equal(sim.read(f), orc.read(f)), f
+    if c.num_sub == 0:
+        return c
+    rho, rho_o = sim.read(Field.SUB_DENSITY), orc.read(Field.SUB_DENSITY)
+    assert np.max(np.abs(rho - rho_o) / rho_o) <= 1e-5
+    p, p_o = sim.read(Field.SUB_PRESSURE), orc.read(Field.SUB_PRESSURE)
+    assert np.max(np.abs(p - p_o)) <= 1e-5 * np.abs(rho_o).max() * 3.0 + 1e-3
+    _check_sub_accel(sim, orc)
+    assert np.max(np.abs(sim.read(Field.SUB_POSITION) - orc.read(Field.SUB_POSITION))) <= 2e-6
+    # sub sort: keys ascending, ties in sub-list order (split parents first,
+    # then boundary parents, children in order: the reference's stable sort)
+    sk = sim.read(Field.SUB_SORTED_KEYS)
+    sids = sim.read(Field.SUB_SORTED_IDS)
+    assert np.all(np.diff(sk) >= 0)
+    listpos = np.empty(int(sids.max()) + 1, np.int64)
+    listpos[sim.read(Field.SUB_IDS)] = np.arange(c.num_sub)
+    lp = listpos[sids]
+    ties = np.diff(sk) == 0
+    assert np.all(np.diff(lp)[ties] > 0)
+    # same order as the oracle except where FP32 and FP64 seeded positions
+    # straddle a sub-cell face
+    assert np.mean(sk == orc.read(Field.SUB_SORTED_KEYS)) > 0.999
+    return c
+
+
+@pytest.fixture(scope="module")
+def dam16k():
+    return S.make_scene("dam16k")
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_two_scale_reference_slab_one_step(dam16k, placement):
+    """The reference's own split region (x in [0.2, 0.5], cu:947) on the
+    16K dam break, both seeding rules."""
+    sc = dam16k.with_positions(dam16k.positions)
+    sc.params.sub_placement = placement
+    sim = S.ParticleSimulation(sc).Init()
+    sim.AdvanceTwoScale(1)
+    sim.sync()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1, two_scale=True)
+    c = _compare_two_scale(sim, orc, sc.n)
+    assert c.num_split > 1000 and c.num_boundary > 100
+    # the coarse step is unchanged by the sub pass
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
+
+
+def test_two_scale_state_history_over_steps(dam16k):
+    sc = dam16k.with_positions(dam16k.positions)
+    sim = S.ParticleSimulation(sc).Init()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    for _ in range(3):
+        sim.AdvanceTwoScale(5)
+        orc.step(5, two_scale=True)
+        sim.sync()
+        st, st_o = sim.read(Field.STATE), orc.read(Field.STATE)
+        assert np.mean(st == st_o) > 0.999
+        c, co = sim.counts(), orc.counts()
+        assert abs(int(c.num_split) - int(co.num_split)) <= 0.002 * co.num_split + 2
+
+
+def test_two_scale_config_full_size():
+    sc = S.make_scene("two_scale")
+    sim = S.ParticleSimulation(sc).Init()
+    sim.AdvanceTwoScale(1)
+    sim.sync()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1, two_scale=True)
+    c = _compare_two_scale(sim, orc, sc.n)
+    print(f"two-scale config: split {c.num_split}, boundary {c.num_boundary}, subs {c.num_sub}")
+    assert c.num_sub > 40000
+
+
+def test_two_scale_then_plain_steps(dam16k):
+    """Plain steps after two-scale steps keep the state bytes and stay in
+    parity."""
+    sc = dam16k.with_positions(dam16k.positions)
+    sim = S.ParticleSimulation(sc).Init()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    sim.AdvanceTwoScale(2)
+    orc.step(2, two_scale=True)
+    sim.Advance(2)
+    orc.step(2)
+    sim.sync()
+    assert np.array_equal(sim.read(Field.STATE), orc.read(Field.STATE))
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-4
+
+
+def test_two_scale_capacity_exceeded_is_reported(dam16k):
+    sc = dam16k.with_positions(dam16k.positions)
+    sc.params.max_sub_particles = 800
+    sim = S.ParticleSimulation(sc).Init()
+    sim.AdvanceTwoScale(1)
+    with pytest.raises(S.SphError) as e:
+        sim.sync()
+    assert e.value.status == S.Status.CAPACITY_EXCEEDED
+    c = sim.counts()
+    assert c.num_sub <= 800

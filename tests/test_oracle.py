@@ -1,0 +1,449 @@
+"""Pins the CPU oracle (no GPU): each check is independent of the oracle's
+own formulas -- closed forms, brute force, invariants and known answers --
+chosen so that a plausible mistake (a dropped term, a wrong sign or index,
+FP64 where the reference computes FP32) fails one of them.
+"""
+import numpy as np
+import pytest
+
+import cudasphnew_amd as S
+from cudasphnew_amd import Field
+from oracle import oracle as O
+from tests.helpers import lattice_positions
+
+G = float(np.float32(9.81))   # the parameter is a float
+
+
+@pytest.fixture(scope="module")
+def base():
+    return S.make_scene("dam16k")
+
+
+def flat_boundary(sc, distance=None, density=0.0, viscosity=0.0):
+    """Same grid geometry, synthetic fields (default: far from any wall)."""
+    shape = sc.distance.shape
+    r0 = sc.desc.rest_distance
+    d = np.full(shape, -r0, np.float32) if distance is None else distance.astype(np.float32)
+    out = S.Scene(sc.desc, S.SphParams.from_buffer_copy(sc.params), sc.positions, d,
+                  np.full(shape, density, np.float32), np.full(shape, viscosity, np.float32))
+    return out
+
+
+def grid_coords(sc):
+    nz, ny, nx = sc.distance.shape
+    o = np.array(sc.desc.boundary_origin[:], np.float64)
+    sp = float(sc.desc.boundary_spacing)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return o[0] + sp * i, o[1] + sp * j, o[2] + sp * k
+
+
+# ---- cell keys, sort, cell ranges ------------------------------------------
+
+def test_keys_are_the_cell_containing_the_particle(base):
+    p = base.params
+    rng = np.random.default_rng(0)
+    pos = rng.uniform(0.0, 0.64, size=(20000, 3)).astype(np.float32)
+    keys, flags = O.keys(p, pos, real=1)
+    assert flags == 0
+    nx, ny = p.grid_dim[0], p.grid_dim[1]
+    c = np.stack([keys % nx, (keys // nx) % ny, keys // (nx * ny)], 1)
+    o = np.array(p.grid_origin[:], np.float64)
+    lo = o + c * p.grid_spacing
+    eps = 1e-6
+    assert np.all(pos >= lo - eps) and np.all(pos < lo + p.grid_spacing + eps)
+
+
+def test_keys_use_fp32_arithmetic(base):
+    """(int)((x - o)/d) evaluated in float32 (compute_particle_hash,
+    particle_simulation.cu:101-107): FP64 evaluation differs for points just
+    below a cell face."""
+    p = base.params
+    o = np.float32(p.grid_origin[0])
+    d = np.float32(p.grid_spacing)
+    faces = o + d * np.arange(3, 30, dtype=np.float32)
+    below = np.nextafter(faces, np.float32(-1))
+    pos = np.stack([below, np.full_like(below, 0.3), np.full_like(below, 0.3)], 1).astype(np.float32)
+    k32, _ = O.keys(p, pos, real=1)
+    ref = ((pos[:, 0] - o) / d).astype(np.float32).astype(np.int64)   # numpy float32 IEEE
+    assert np.array_equal(k32 % p.grid_dim[0], ref)
+    k64, _ = O.keys(p, pos, real=0)
+    assert not np.array_equal(k32, k64)
+
+
+def test_keys_out_of_domain_definition(base):
+    p = base.params
+    pos = np.array([[-0.5, 0.3, 0.3], [0.3, 0.3, 5.0], [0.3, 0.3, 0.3]], np.float32)
+    keys, flags = O.keys(p, pos, real=1)
+    assert flags & O.FLAG_OUT_OF_DOMAIN
+    nx, ny, nz = p.grid_dim
+    # below the grid -> coordinate 0; beyond -> wrapped as the reference's "% dim"
+    assert keys[0] % nx == 0
+    cz = int(np.float32((np.float32(5.0) - np.float32(p.grid_origin[2])) / np.float32(p.grid_spacing)))
+    assert keys[1] // (nx * ny) == cz % nz
+
+
+def test_sort_is_stable_by_key_then_id(base):
+    rng = np.random.default_rng(1)
+    n = 50000
+    ncell = int(np.prod(base.params.grid_dim[:]))
+    keys = rng.integers(0, ncell // 40, size=n).astype(np.int32)
+    sk, si, cs, ce = O.sort_cells(base.params, keys)
+    ref = np.argsort(keys, kind="stable")
+    assert np.array_equal(si, ref)
+    assert np.array_equal(sk, keys[ref])
+    u, first, cnt = np.unique(sk, return_index=True, return_counts=True)
+    assert np.array_equal(cs[u], first) and np.array_equal(ce[u], first + cnt)
+    empty = np.ones(ncell, bool)
+    empty[u] = False
+    assert np.all(cs[empty] == -1) and np.all(ce[empty] == -1)
+
+
+# ---- density -------------------------------------------------------------------
+
+def test_density_known_answer_perfect_lattice(base):
+    """Interior particle of a cubic lattice with spacing s = h/2: the lattice
+    vectors with |v|^2 <= 4 are 1, 6, 12, 8, 6 for |v|^2 = 0..4, so
+    sum (h^2 - r^2)^3 = s^6 * (64 + 6*27 + 12*8 + 8*1) = 330 s^6."""
+    sc = flat_boundary(base)
+    s, h, m = 0.01, 0.02, float(sc.params.particle_mass)
+    pos = lattice_positions((9, 9, 9), s, (0.28, 0.28, 0.28))
+    orc = O.OracleSim(sc, real=0, positions=pos)
+    orc.step(1)
+    rho = orc.read(Field.DENSITY)
+    centre = 4 + 9 * (4 + 9 * 4)
+    poly6 = float(sc.params.poly6)
+    # h is the float 0.02f, a hair below 2s: the |v|^2 = 4 neighbours sit at
+    # r = 2s > h and drop out; their (h^2 - 4 s^2)^3 term is ~1e-21 anyway
+    hf = float(np.float32(h))
+    counts = {0: 1, 1: 6, 2: 12, 3: 8}
+    expect = m * poly6 * sum(c * (hf * hf - s * s * k) ** 3 for k, c in counts.items())
+    assert rho[centre] == pytest.approx(expect, rel=1e-12)
+    assert expect == pytest.approx(m * poly6 * 330 * s ** 6, rel=1e-6)
+    assert 1000 < expect < 1020
+
+
+def test_density_matches_brute_force(base):
+    sc = flat_boundary(base)
+    rng = np.random.default_rng(2)
+    pos = 0.25 + 0.08 * rng.random((1500, 3))
+    orc = O.OracleSim(sc, real=0, positions=pos)
+    orc.step(1)
+    assert np.allclose(orc.read(Field.DENSITY), O.density_brute(sc, pos), rtol=1e-12)
+
+
+def test_wall_density_is_added(base):
+    """rho = m sum W + W_wall(x) (particle_simulation.cu:240-248) with a
+    constant wall-density field."""
+    sc0 = flat_boundary(base)
+    sc1 = flat_boundary(base, density=12.5)
+    pos = 0.3 + 0.05 * np.random.default_rng(3).random((300, 3))
+    a = O.OracleSim(sc0, positions=pos)
+    b = O.OracleSim(sc1, positions=pos)
+    a.step(1)
+    b.step(1)
+    assert np.allclose(b.read(Field.DENSITY) - a.read(Field.DENSITY), 12.5, atol=1e-9)
+    assert np.allclose(b.read(Field.PRESSURE) - a.read(Field.PRESSURE), 3.0 * 12.5, atol=1e-8)
+
+
+# ---- acceleration ----------------------------------------------------------------
+
+def test_acceleration_matches_brute_force(base):
+    sc = flat_boundary(base)
+    rng = np.random.default_rng(4)
+    pos = 0.25 + 0.08 * rng.random((800, 3))
+    vel = 0.2 * rng.standard_normal((800, 3))
+    orc = O.OracleSim(sc, real=0, positions=pos, velocities=vel)
+    orc.step(1)
+    acc = orc.read(Field.ACCEL)
+    ref = O.accel_brute(sc, pos, vel, orc.read(Field.DENSITY))
+    assert np.allclose(acc, ref, rtol=1e-10, atol=1e-9)
+
+
+def test_pair_forces_conserve_momentum(base):
+    """Pressure and viscosity terms are antisymmetric per pair, so with no
+    gravity, no tension and no wall: sum_i m a_i = 0."""
+    sc = flat_boundary(base)
+    sc.params.gravity = 0.0
+    sc.params.norm_thresh = 1e30
+    rng = np.random.default_rng(5)
+    pos = 0.25 + 0.08 * rng.random((1000, 3))
+    vel = 0.3 * rng.standard_normal((1000, 3))
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)
+    assert np.abs(a.sum(0)).max() <= 1e-10 * np.abs(a).sum(0).max()
+    # a wrong sign on either term breaks it: the net force of each term alone
+    # is not zero unless the pairing is right (checked against brute force)
+    assert np.abs(a).max() > 1.0
+
+
+def test_uniform_lattice_interior_feels_only_gravity(base):
+    sc = flat_boundary(base)
+    pos = lattice_positions((11, 11, 11), 0.01, (0.25, 0.25, 0.25))
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)
+    c = 5 + 11 * (5 + 11 * 5)
+    assert a[c] == pytest.approx([0.0, -G, 0.0], abs=1e-9)
+
+
+def test_viscosity_damps_relative_motion(base):
+    """Two particles approaching each other: the viscosity force on each
+    opposes the relative velocity (laplVisc > 0, (v_j - v_i) sign)."""
+    sc = flat_boundary(base)
+    sc.params.gravity = 0.0
+    sc.params.gas_stiffness = 0.0
+    pos = np.array([[0.30, 0.3, 0.3], [0.31, 0.3, 0.3]])
+    vel = np.array([[1.0, 0, 0], [-1.0, 0, 0]])
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)
+    assert a[0, 0] < 0 < a[1, 0]
+
+
+def test_pressure_repels_compressed_pair(base):
+    sc = flat_boundary(base)
+    sc.params.gravity = 0.0
+    sc.params.rest_density = 1.0          # positive pressure
+    pos = np.array([[0.30, 0.3, 0.3], [0.305, 0.3, 0.3]])
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)
+    assert a[0, 0] < 0 < a[1, 0]
+    assert a[0, 0] == pytest.approx(-a[1, 0], rel=1e-12)
+
+
+# ---- boundary lookups and the wall term ----------------------------------------------
+
+def test_texture_lookup_of_linear_field_is_exact(base):
+    """Trilinear filtering reproduces a linear field at the texel coordinate
+    t = u*dims - 0.5 (normalized coordinates, set_up_3d_float_texture)."""
+    X, Y, Z = grid_coords(base)
+    a, b, c, d = 0.3, -1.2, 0.7, 0.05
+    f = a * X + b * Y + c * Z + d
+    sc = flat_boundary(base, distance=f)
+    rng = np.random.default_rng(6)
+    pts = rng.uniform(0.05, 0.6, size=(500, 3))
+    got = O.tex(sc, 0, pts)
+    o = np.array(base.desc.boundary_origin[:], np.float64)
+    n = np.array(base.desc.boundary_dims[:], np.float64)
+    sp = float(np.float32(base.desc.boundary_spacing))
+    L = np.float32((n - 1).astype(np.float32) * np.float32(base.desc.boundary_spacing)).astype(np.float64)
+    t = (pts - o) / L * n - 0.5                # texel coordinates
+    eff = o + t * sp                           # sampled world point
+    ref = a * eff[:, 0] + b * eff[:, 1] + c * eff[:, 2] + d
+    assert np.allclose(got, ref, atol=2e-7)    # grid values are float32
+
+
+def test_wall_penalty_closed_form(base):
+    """A lone particle near the floor with distance field D = -y (positive in
+    the solid): a_y = |G| (r0 - d)/dt^2 - g with d the distance at the
+    sampled point and G the normalized-coordinate central difference
+    (particle_simulation.cu:407-452)."""
+    X, Y, Z = grid_coords(base)
+    sc = flat_boundary(base, distance=-Y)
+    r0 = float(np.float32(base.desc.rest_distance))
+    y = 0.015
+    pos = np.array([[0.3, y, 0.3]])
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)[0]
+    d = -O.tex(sc, 0, pos)[0]
+    assert d < r0
+    n = base.desc.boundary_dims[1]
+    Ly = float(np.float32(np.float32(n - 1) * np.float32(base.desc.boundary_spacing)))
+    dt = float(np.float32(base.params.time_step))
+    # the +-spacing/L offsets in u are n/(n-1) texels apart, so the
+    # normalized gradient of D = -y has magnitude L n/(n-1) = n * spacing
+    # (float32 node values: the linear field is exact to ~1e-8)
+    assert a[1] == pytest.approx(Ly * n / (n - 1) * (r0 - d) / dt ** 2 - G, rel=1e-6)
+    assert abs(a[0]) < 1e-6 and abs(a[2]) < 1e-6
+
+
+def test_wall_viscosity_term(base):
+    X, Y, Z = grid_coords(base)
+    sc = flat_boundary(base, distance=-Y, viscosity=40.0)
+    pos = np.array([[0.3, 0.015, 0.3]])
+    vel = np.array([[0.5, 0.0, -0.25]])
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1)
+    a = orc.read(Field.ACCEL)[0]
+    rho = orc.read(Field.DENSITY)[0]
+    # F -= v * V (particle_simulation.cu:446-449)
+    assert a[0] == pytest.approx(-0.5 * 40.0 / rho, rel=1e-9)
+    assert a[2] == pytest.approx(0.25 * 40.0 / rho, rel=1e-9)
+
+
+# ---- integration and collision ---------------------------------------------------------
+
+def test_free_fall_symplectic_euler_closed_form(base):
+    sc = flat_boundary(base)
+    pos = np.array([[0.3, 0.5, 0.3]])
+    orc = O.OracleSim(sc, positions=pos)
+    n = 300
+    orc.step(n)
+    dt = float(np.float32(base.params.time_step))
+    g = float(np.float32(base.params.gravity))
+    assert orc.read(Field.POSITION)[0, 1] == pytest.approx(0.5 - g * dt * dt * n * (n + 1) / 2, abs=1e-13)
+    assert orc.read(Field.VELOCITY)[0, 1] == pytest.approx(-g * dt * n, rel=1e-12)
+
+
+def test_collision_projects_onto_surface_and_removes_normal_velocity(base):
+    X, Y, Z = grid_coords(base)
+    sc = flat_boundary(base, distance=-Y)
+    sc.desc = S.SceneDesc.from_buffer_copy(base.desc)
+    sc.desc.rest_distance = 1e-9          # wall penalty off: collision alone
+    sc.params.collision = 1
+    sc.params.gravity = 0.0
+    r0 = float(sc.desc.rest_distance)
+    # the sampled zero level of D = -y sits at y ~ 0.005 (half-texel shift of
+    # the normalized texture coordinates); start 0.7 mm above it, fast enough
+    # to cross it within one step
+    y0 = 0.0057
+    assert O.tex(sc, 0, np.array([[0.3, y0, 0.3]]))[0] < 0
+    pos = np.array([[0.3, y0, 0.3]])
+    vel = np.array([[0.2, -10.0, 0.0]])
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1)
+    x = orc.read(Field.POSITION)[0]
+    v = orc.read(Field.VELOCITY)[0]
+    assert np.abs(orc.read(Field.ACCEL)[0]).max() < 1e-9     # no wall force
+    # restitution 0: the normal component is removed, tangential kept
+    assert abs(v[1]) < 1e-9
+    # the contact point x - D n lies on the zero level of the sampled field
+    # up to the n/(n-1) stretch of the normalized texel mapping:
+    # D_after = -D_before / (n - 1)
+    dt = float(np.float32(base.params.time_step))
+    x_pre = pos[0] + dt * vel[0]
+    D_before = O.tex(sc, 0, x_pre[None])[0]
+    assert D_before > 0
+    D_after = O.tex(sc, 0, x[None])[0]
+    n = base.desc.boundary_dims[1]
+    assert D_after == pytest.approx(-D_before / (n - 1), rel=1e-4)
+    assert x[0] == pytest.approx(pos[0, 0] + dt * vel[0, 0], abs=1e-12)
+    del r0
+
+
+# ---- boundary builder --------------------------------------------------------------------
+
+def test_builder_taken_nodes_and_clamp(base):
+    d = S.config_desc("dam16k")
+    D, W, V = O.scene_boundary(d)
+    r0 = d.rest_distance
+    X, Y, Z = grid_coords(base)
+    half = 0.5 * float(np.float32(0.64))
+    q = np.abs(np.stack([X, Y, Z], -1) - half) - half
+    sdf = np.linalg.norm(np.maximum(q, 0), axis=-1) + np.minimum(q.max(-1), 0)
+    taken = np.abs(sdf) < d.boundary_support - 1e-7
+    assert np.allclose(D[taken], np.clip(sdf[taken], -r0, r0), atol=1e-6)
+    far = np.abs(sdf) > d.boundary_support + 1e-7
+    assert np.all(D[far] == np.float32(-r0))
+    assert np.all(W[far] == 0) and np.all(V[far] == 0)
+    # wall density is largest on the wall and vanishes at depth >= cs inside
+    assert W[taken].max() > 0 and np.all(W >= 0) and np.all(V >= 0)
+
+
+# ---- precision ---------------------------------------------------------------------------
+
+def test_fp32_transcription_tracks_fp64_over_200_steps(base):
+    a = O.OracleSim(base, real=0, threads=8)
+    b = O.OracleSim(base, real=1, threads=8)
+    a.step(200)
+    b.step(200)
+    ra, rb = a.read(Field.DENSITY), b.read(Field.DENSITY)
+    assert abs(ra.mean() - rb.mean()) < 1e-3 * ra.mean()
+    va, vb = a.read(Field.VELOCITY), b.read(Field.VELOCITY)
+    assert abs((va ** 2).sum() - (vb ** 2).sum()) < 1e-2 * (va ** 2).sum()
+
+
+# ---- two-scale pass ---------------------------------------------------------------------------
+
+def two_scale_scene(base, placement=0):
+    sc = flat_boundary(base)
+    p = sc.params
+    p.split_lo[:] = [0.30, -1e30, -1e30]
+    p.split_hi[:] = [0.34, 1e30, 1e30]
+    p.sub_placement = placement
+    pos = lattice_positions((10, 6, 6), 0.01, (0.26, 0.26, 0.26))
+    pos = pos + 0.001 * np.random.default_rng(8).uniform(-1, 1, pos.shape)
+    return sc, pos
+
+
+@pytest.mark.parametrize("placement", [0, 1])
+def test_two_scale_classification_and_lists(base, placement):
+    sc, pos = two_scale_scene(base, placement)
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1, two_scale=True)
+    c = orc.counts()
+    x = pos[:, 0]
+    split = (x >= 0.30) & (x <= 0.34)
+    d2 = ((pos[:, None, :] - pos[None, split, :]) ** 2).sum(-1)
+    near = (d2 <= float(np.float32(sc.params.compact_support)) ** 2).any(1)
+    boundary = near & ~split
+    assert c.num_split == split.sum() and c.num_boundary == boundary.sum()
+    assert c.num_default == (~split & ~near).sum()
+    assert np.array_equal(orc.read(Field.SPLIT_IDS), np.nonzero(split)[0])
+    assert np.array_equal(orc.read(Field.BOUNDARY_IDS), np.nonzero(boundary)[0])
+    st = orc.read(Field.STATE)
+    assert np.all((st & 3)[split] == 3) and np.all((st & 3)[boundary] == 2)
+    assert np.all((st & 3)[~split & ~near] == 0)
+    sid = orc.read(Field.SUB_IDS)
+    parents = np.concatenate([np.nonzero(split)[0], np.nonzero(boundary)[0]])
+    assert np.array_equal(sid, (8 * parents[:, None] + np.arange(8)).ravel())
+    # state history: one more step shifts the two bits up
+    orc.step(1, two_scale=True)
+    st2 = orc.read(Field.STATE)
+    assert np.all((st2 >> 2)[split] & 3 == 3)
+
+
+def test_two_scale_children_geometry(base):
+    """Reference seeding: 8 children on the corners of a cube centred on the
+    parent, at distance r = (3m/(4 pi rho))^(1/3) (radius of the parent's
+    volume sphere)."""
+    sc, pos = two_scale_scene(base, 0)
+    sc.params.time_step_sub = 1e-12          # read the seeded positions
+    sc.params.gravity = 0.0
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1, two_scale=True)
+    sid = orc.read(Field.SUB_IDS)
+    sx = orc.read(Field.SUB_POSITION).reshape(-1, 8, 3)
+    rho = orc.read(Field.DENSITY)
+    parents = sid[::8] // 8
+    centre = sx.mean(1)
+    # parents' positions are pre-integration: compare against the step's input
+    assert np.allclose(centre, pos[parents], atol=1e-9)
+    r = np.linalg.norm(sx - pos[parents][:, None, :], axis=-1)
+    m = float(np.float32(sc.params.particle_mass))
+    expect = (3 * m / (4 * np.pi * rho[parents])) ** (1 / 3)
+    assert np.allclose(r, expect[:, None], rtol=1e-7)
+
+
+def test_two_scale_projection_and_sub_density_brute_force(base):
+    sc, pos = two_scale_scene(base, 1)
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1, two_scale=True)
+    c = orc.counts()
+    nsp = 8 * c.num_split
+    sx = orc.read(Field.SUB_POSITION)
+    srho = orc.read(Field.SUB_DENSITY)
+    rho = orc.read(Field.DENSITY)
+    # recompute seeded positions (lattice placement: parent + h/4 (+-1,+-1,+-1))
+    sid = orc.read(Field.SUB_IDS)
+    par = sid // 8
+    ch = sid % 8
+    sgn = np.stack([np.where(ch & 4, -1, 1), np.where(ch & 2, -1, 1), np.where(ch & 1, -1, 1)], 1)
+    h = float(np.float32(sc.params.compact_support))
+    seeded = pos[par] + sgn * h / 4
+    hs = float(np.float32(sc.params.compact_support_sub))
+    # boundary subs: unweighted mean density of coarse particles within h_sub
+    for k in range(nsp, c.num_sub, 7):
+        d2 = ((pos - seeded[k]) ** 2).sum(1)
+        assert srho[k] == pytest.approx(rho[d2 <= hs * hs].mean(), rel=1e-12)
+    # split subs: poly6_sub sum over all sub-particles within h_sub
+    poly = float(np.float32(sc.params.poly6_sub))
+    ms = float(np.float32(sc.params.sub_particle_mass))
+    for k in range(0, nsp, 5):
+        d2 = ((seeded - seeded[k]) ** 2).sum(1)
+        w = np.maximum(hs * hs - d2, 0.0) ** 3
+        assert srho[k] == pytest.approx(ms * poly * w.sum(), rel=1e-12)
+    del sx
