@@ -69,6 +69,23 @@ STAGE_BYTES_PER_PARTICLE = {
 }
 
 
+def copy_probe(nbytes: int) -> None:
+    """One device-to-device copy of known size through the HIP runtime the
+    library itself uses (libamdhip64), to calibrate rocprof byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("copy probe: hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)       # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written")
+
+
 def cpu_baseline(scene, seconds: float, threads: int):
     from oracle import oracle as O
     sim = O.OracleSim(scene, real=1, threads=threads)
@@ -96,6 +113,8 @@ def main(argv=None) -> int:
     ap.add_argument("--config", default="dam1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="run this many steps with no timing events and exit (for counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
     a = ap.parse_args(argv)
@@ -107,12 +126,7 @@ def main(argv=None) -> int:
     import cudasphnew_amd as S
 
     if a.copy_probe:
-        import torch   # plumbing only: a copy kernel of known size
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy probe: 268435456 bytes read, 268435456 bytes written")
+        copy_probe(256 << 20)
     t0 = time.time()
     sc = S.make_scene(a.config)
     log(f"scene {a.config}: {sc.n} particles, grid {tuple(sc.params.grid_dim)}, "
@@ -121,6 +135,11 @@ def main(argv=None) -> int:
     sim.Advance(a.warmup)
     sim.sync()
     log(f"warm-up {a.warmup} steps done")
+    if a.profile_steps:
+        sim.Advance(a.profile_steps)
+        sim.sync()
+        log(f"profile: {a.profile_steps} steps done")
+        return 0
     sim.enable_timing(True)
     ms = sim.time_steps(a.steps)
     stages = sim.stage_times()

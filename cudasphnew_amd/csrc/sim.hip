@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -182,8 +183,14 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
     key_bits = 1;
     while (key_bits < 32 && (1ull << key_bits) < (uint64_t)total_cells) ++key_bits;
 
-    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess)
+    // One stream per handle.  SPH_STREAM=null puts the handle on the null
+    // stream instead (for profilers that only attribute counters there).
+    const char* sv = std::getenv("SPH_STREAM");
+    if (sv && std::strcmp(sv, "null") == 0) {
+        stream = nullptr;
+    } else if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
         return fail(SPH_DEVICE_ERROR, "hipStreamCreate failed");
+    }
     sph_status st;
 #define SPH_TRY(x) do { if ((st = (x)) != SPH_OK) return st; } while (0)
     SPH_TRY(alloc(&d_scenes, S));
