@@ -117,6 +117,9 @@ def main(argv=None) -> int:
                     help="run this many steps with no timing events and exit (for counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--tuning", default="auto",
+                    help="density/force kernel variants: 'auto' (autotune after the warm-up) or "
+                         "'D,F,M,C' = density lanes, force lanes, hit masks (0/1), hit-list capacity")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -132,9 +135,17 @@ def main(argv=None) -> int:
     log(f"scene {a.config}: {sc.n} particles, grid {tuple(sc.params.grid_dim)}, "
         f"boundary {sc.boundary_dims} ({time.time() - t0:.1f}s)")
     sim = S.ParticleSimulation(sc).Init()
-    sim.Advance(a.warmup)
+    sim.Advance(max(a.warmup, 1))
     sim.sync()
     log(f"warm-up {a.warmup} steps done")
+    if a.tuning == "auto":
+        tuning = sim.autotune(reps=3)
+    else:
+        sim.set_tuning(*(int(v) for v in a.tuning.split(",")))
+        tuning = sim.get_tuning()
+    sim.Advance(2)
+    sim.sync()
+    log(f"kernel variants: {tuning}")
     if a.profile_steps:
         sim.Advance(a.profile_steps)
         sim.sync()
@@ -187,7 +198,8 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": WORKLOADS[a.config], "form": "plain", "particles": n},
+        "config": {"workload": WORKLOADS[a.config], "form": "plain", "particles": n,
+                   "kernel_variants": tuning},
         "stage_ms": {k: round(v, 5) for k, v in stages.items() if v > 0},
         "roofline": roofline,
         "cpu_baseline": cpu,

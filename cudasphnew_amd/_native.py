@@ -120,6 +120,11 @@ class SphCounts(C.Structure):
     ]
 
 
+class SphTuning(C.Structure):
+    _fields_ = [("density_lanes", C.c_int32), ("force_lanes", C.c_int32),
+                ("hit_masks", C.c_int32), ("hit_list", C.c_int32)]
+
+
 class SphSolid(C.Structure):
     _fields_ = [("kind", C.c_int32), ("center", f3), ("a", f3)]
 
@@ -181,6 +186,9 @@ SIGNATURES = [
     ("sph_stage_times", _st, [_vp, fptr, C.POINTER(C.c_int32)]),
     ("sph_stage_name", C.c_char_p, [C.c_int32]),
     ("sph_time_steps", _st, [_vp, C.c_int32, C.c_int32, fptr]),
+    ("sph_set_tuning", _st, [_vp, C.POINTER(SphTuning)]),
+    ("sph_get_tuning", _st, [_vp, C.POINTER(SphTuning)]),
+    ("sph_autotune", _st, [_vp, C.c_int32, C.POINTER(SphTuning)]),
     ("sph_batch_create", _st, [C.c_int32, C.POINTER(SphParams), C.POINTER(SphScene), C.POINTER(_vp)]),
     ("sph_batch_destroy", _st, [_vp]),
     ("sph_batch_step", _st, [_vp, C.c_int32]),
@@ -190,6 +198,9 @@ SIGNATURES = [
     ("sph_batch_enable_timing", _st, [_vp, C.c_int32]),
     ("sph_batch_stage_times", _st, [_vp, fptr, C.POINTER(C.c_int32)]),
     ("sph_batch_use_graph", _st, [_vp, C.c_int32]),
+    ("sph_batch_set_tuning", _st, [_vp, C.POINTER(SphTuning)]),
+    ("sph_batch_get_tuning", _st, [_vp, C.POINTER(SphTuning)]),
+    ("sph_batch_autotune", _st, [_vp, C.c_int32, C.POINTER(SphTuning)]),
     ("sph_batch_num_particles", C.c_uint64, [_vp]),
     ("sph_batch_last_error_message", C.c_char_p, [_vp]),
     ("sph_scene_config", _st, [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(SceneDesc)]),

@@ -90,6 +90,17 @@ __device__ __forceinline__ uint32_t sub_cell_key(const DevScene& S, float x, flo
     return (uint32_t)(S.snx * (S.sny * cz + cy) + cx);
 }
 
+// XCD-aware block order.  The 8 XCDs of an MI355X receive workgroups
+// round-robin (block b runs on XCD b % 8), each XCD with its own L2.  Giving
+// XCD x the contiguous logical blocks [x*G/8, (x+1)*G/8) keeps neighbouring
+// particle tiles -- which share their neighbour rows -- in one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G) {
+    constexpr uint32_t X = 8;
+    uint32_t per = G / X, rem = G % X;
+    uint32_t x = b % X, i = b / X;
+    return x * per + min(x, rem) + i;
+}
+
 // Scene of a particle slot: scenes occupy fixed contiguous slot ranges in
 // both storage and sorted order (their keys are disjoint, increasing ranges).
 __device__ __forceinline__ int scene_of(const uint32_t* __restrict__ begins, int nscenes, uint32_t s) {

@@ -87,8 +87,16 @@ struct Engine {
     hipGraphExec_t graph_exec = nullptr;
 
     TwoScale* ts = nullptr;
+    bool last_two_scale = false;
+
+    // kernel variants of the density / force passes
+    sph_tuning tuning = {1, 1, 0, 48};
+    uint64_t* masks = nullptr;   // [9][N] per-row force-neighbour masks
 
     ~Engine();
+    void drop_graph();
+    sph_status set_tuning(const sph_tuning& t);
+    sph_status autotune(int reps, sph_tuning* out);
     sph_status init(int S, const sph_params* P, const sph_scene* scenes);
     sph_status step(int n, bool two_scale);
     sph_status sync();

@@ -32,13 +32,20 @@ struct StepBuffers {
     uint32_t* slot_of_oid = nullptr;
     int2* range = nullptr;       // cell -> [start, end) in sorted order
     uint32_t* status = nullptr;
+    // per-row force-neighbour masks [9][n] written by the density pass and
+    // read by the force pass; null = neighbour loops without masks
+    uint64_t* masks = nullptr;
 };
 
 void launch_hash(hipStream_t st, const StepBuffers& b);
 void launch_cells_clear(hipStream_t st, const StepBuffers& b);
 void launch_cells(hipStream_t st, const StepBuffers& b);
 void launch_reorder(hipStream_t st, const StepBuffers& b);
-void launch_density(hipStream_t st, const StepBuffers& b, int block);
-void launch_force(hipStream_t st, const StepBuffers& b, int block);
+// lanes: lanes per particle of the neighbour loops (1 = one thread per
+// particle; 2, 4, 8, 16 = wavefront-cooperative variants).  With b.masks set
+// the hit-mask pair is launched instead and the force's lanes value selects
+// its per-thread hit-list capacity (32, 48 or 64).
+void launch_density(hipStream_t st, const StepBuffers& b, int lanes);
+void launch_force(hipStream_t st, const StepBuffers& b, int lanes);
 
 }  // namespace sph

@@ -98,119 +98,92 @@ __device__ __forceinline__ void row_run(const int2* __restrict__ range, uint32_t
     }
 }
 
-// compute_particle_density_pressure (cu:196-256, 1385-1419).  Density goes
-// to vel.w; pressure k(rho - rho0) is recomputed where needed.
-__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                    const int2* __restrict__ range, uint32_t n,
-                                                    const DevScene* __restrict__ scenes,
-                                                    const uint32_t* __restrict__ begins, int nscenes,
-                                                    const float4* __restrict__ grid4) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
-    float4 xi = pos[s];
-    const float h = S.h, h2 = S.h2;
-    int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
-    int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
-    int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
-    float sum = 0.0f;
-    for (int k = lz; k <= hz; ++k) {
-        for (int j = ly; j <= hy; ++j) {
-            uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
-            int b, e;
-            row_run(range, rowbase, lx, hx, b, e);
-            for (int q = b; q < e; ++q) {
-                float4 xj = pos[q];
-                float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                float d = fmaxf(h2 - r2, 0.0f);
-                sum += d * d * d;
-            }
-        }
-    }
-    float rho = sum * (S.poly6 * S.mass);
-    float tx, ty, tz;
-    texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
-    float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
-    rho += g.y;
-    reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+// Neighbour range of a particle on the coarse grid (compute_grid_coordinate,
+// cu:1238-1258) and the nine x-row runs of it.
+struct Rows {
+    int lx, hx, ly, hy, lz, hz;
+};
+
+__device__ __forceinline__ Rows neighbour_rows(const DevScene& S, float4 xi) {
+    const float h = S.h;
+    Rows r;
+    r.lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx); r.hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
+    r.ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny); r.hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
+    r.lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz); r.hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+    return r;
 }
 
-// compute_particle_acceleration_ifsurf (cu:315-474, 1459-1556) fused with
-// integrate_euler (cu:632-665), the collision response (collision_handling,
-// cu:742-834, driven by the distance grid) and the next step's cell key.
-// Writes the next storage order (= this step's sorted order).
-__global__ __launch_bounds__(kBlock) void k_force(
-    const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
-    const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
-    const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
-    const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
-    float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
-    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
-    const float4 xi = pos[s];
-    const float4 vi = vel[s];
+// Slot run of row (dk, dj) of the 3x3 row set; empty when the row is outside
+// the clamped range (e.g. at the grid edge, or the reference's wrapped upper
+// coordinate).
+__device__ __forceinline__ void row_slots(const int2* __restrict__ range, const DevScene& S, const Rows& R, int dk,
+                                          int dj, int& b, int& e) {
+    const int k = R.lz + dk, j = R.ly + dj;
+    b = 0;
+    e = 0;
+    if (k > R.hz || j > R.hy) return;
+    uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
+    row_run(range, rowbase, R.lx, R.hx, b, e);
+    if (e < b) { b = 0; e = 0; }
+}
+
+// Pair terms of the acceleration (cu:1486-1555) accumulated without the
+// constant factors: m, gradSpiky, mu, laplVisc, gradPoly6, laplPoly6 are
+// applied once in force_finish.
+struct PairSums {
+    float fpx, fpy, fpz, fvx, fvy, fvz, cgx, cgy, cgz, cl;
+};
+
+__device__ __forceinline__ void pair_terms(PairSums& a, float4 xi, float4 vi, float rhoi, float Ai, float h, float h2,
+                                           float stiff, float kr0, float4 xj, float4 vj) {
+    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float inv_r = __builtin_amdgcn_rsqf(r2);
+    float rn = r2 * inv_r;
+    float invj = __builtin_amdgcn_rcpf(vj.w);
+    float pj = stiff * vj.w - kr0;
+    float hr = h - rn;
+    float qd = h2 - r2;
+    float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
+    a.fpx += cp * dx; a.fpy += cp * dy; a.fpz += cp * dz;
+    float wv = invj * hr;
+    a.fvx += wv * (vj.x - vi.x); a.fvy += wv * (vj.y - vi.y); a.fvz += wv * (vj.z - vi.z);
+    float wc = invj * qd * qd;
+    a.cgx += wc * dx; a.cgy += wc * dy; a.cgz += wc * dz;
+    a.cl += invj * qd * (3.0f * h2 - 7.0f * r2);
+}
+
+// Wall penalty + wall viscosity or surface tension (cu:407-466), a = F/rho - g
+// (cu:470-473), symplectic Euler (integrate_euler, cu:632-665), the
+// grid-driven collision response (collision_handling, cu:742-834) and the
+// next step's cell key; writes the next storage order (this step's sorted
+// order).
+__device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, float4 xi, float4 vi, const PairSums& a,
+                                             const float4* __restrict__ grid4, const float* __restrict__ gridd,
+                                             const uint32_t* __restrict__ oid_b, const uint8_t* __restrict__ state_b,
+                                             float4* __restrict__ acc, float4* __restrict__ pos_out,
+                                             float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out,
+                                             uint8_t* __restrict__ state_out, uint32_t* __restrict__ key_out,
+                                             uint32_t* __restrict__ status) {
     const float rhoi = vi.w;
-    const float h = S.h, h2 = S.h2;
-    const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
-    const float kr0 = S.stiff * S.rho0;
-    int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
-    int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
-    int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
-    float fpx = 0, fpy = 0, fpz = 0;   // sum (A_i + rho_i p_j/rho_j^2) (h-r)^2 / r * r_ij
-    float fvx = 0, fvy = 0, fvz = 0;   // sum (h-r)/rho_j * (v_j - v_i)
-    float cgx = 0, cgy = 0, cgz = 0;   // sum (h^2-r^2)^2/rho_j * r_ij
-    float cl = 0;                      // sum (h^2-r^2)(3h^2-7r^2)/rho_j
-    for (int k = lz; k <= hz; ++k) {
-        for (int j = ly; j <= hy; ++j) {
-            uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
-            int b, e;
-            row_run(range, rowbase, lx, hx, b, e);
-            for (int q = b; q < e; ++q) {
-                float4 xj = pos[q];
-                float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= h2 && r2 > 0.0f) {
-                    float4 vj = vel[q];
-                    float inv_r = __builtin_amdgcn_rsqf(r2);
-                    float rn = r2 * inv_r;
-                    float invj = __builtin_amdgcn_rcpf(vj.w);
-                    float pj = S.stiff * vj.w - kr0;
-                    float hr = h - rn;
-                    float qd = h2 - r2;
-                    float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
-                    fpx += cp * dx; fpy += cp * dy; fpz += cp * dz;
-                    float wv = invj * hr;
-                    fvx += wv * (vj.x - vi.x); fvy += wv * (vj.y - vi.y); fvz += wv * (vj.z - vi.z);
-                    float wc = invj * qd * qd;
-                    cgx += wc * dx; cgy += wc * dy; cgz += wc * dz;
-                    cl += invj * qd * (3.0f * h2 - 7.0f * r2);
-                }
-            }
-        }
-    }
     const float cpf = -S.mass * S.grad_spiky, cvf = S.mu * S.mass * S.lapl_visc;
-    float Fx = cvf * fvx + cpf * fpx, Fy = cvf * fvy + cpf * fpy, Fz = cvf * fvz + cpf * fpz;
+    float Fx = cvf * a.fvx + cpf * a.fpx, Fy = cvf * a.fvy + cpf * a.fpy, Fz = cvf * a.fvz + cpf * a.fpz;
     float tx, ty, tz;
     texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
     float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
     float distWall = -g.x;
     if (distWall < S.r0) {
-        // wall penalty and wall viscosity (cu:407-452)
         float coeff = rhoi / (S.dt * S.dt) * (S.r0 - distWall);
         float3 gn = dist_grad_normalized(gridd, S, tx, ty, tz);
         Fx -= coeff * gn.x + vi.x * g.z;
         Fy -= coeff * gn.y + vi.y * g.z;
         Fz -= coeff * gn.z + vi.z * g.z;
     } else {
-        // surface tension (cu:454-466)
         float cm = S.mass * S.grad_poly6;
-        float gx = cm * cgx, gy = cm * cgy, gz = cm * cgz;
+        float gx = cm * a.cgx, gy = cm * a.cgy, gz = cm * a.cgz;
         float gnorm = sqrtf(gx * gx + gy * gy + gz * gz);
         if (gnorm > S.norm_thresh) {
-            float fc = S.sigma * (S.mass * S.lapl_poly6 * cl) / gnorm;
+            float fc = S.sigma * (S.mass * S.lapl_poly6 * a.cl) / gnorm;
             Fx -= fc * gx;
             Fy -= fc * gy;
             Fz -= fc * gz;
@@ -218,7 +191,6 @@ __global__ __launch_bounds__(kBlock) void k_force(
     }
     float ax = Fx / rhoi, ay = Fy / rhoi - S.gravity, az = Fz / rhoi;
     acc[s] = make_float4(ax, ay, az, 0.0f);
-    // integrate_euler: v += dt a; x += dt v
     float vx = vi.x + S.dt * ax, vy = vi.y + S.dt * ay, vz = vi.z + S.dt * az;
     float px = xi.x + S.dt * vx, py = xi.y + S.dt * vy, pz = xi.z + S.dt * vz;
     if (S.collision) {
@@ -256,6 +228,380 @@ __global__ __launch_bounds__(kBlock) void k_force(
     if (state_out) state_out[s] = state_b[s];
 }
 
+__device__ __forceinline__ float wall_density(const DevScene& S, const float4* __restrict__ grid4, float4 xi) {
+    float tx, ty, tz;
+    texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
+    return lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz)).y;
+}
+
+// compute_particle_density_pressure (cu:196-256, 1385-1419), one thread per
+// particle.  Density goes to vel.w; pressure k(rho - rho0) is recomputed
+// where needed.
+__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                    const int2* __restrict__ range, uint32_t n,
+                                                    const DevScene* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ begins, int nscenes,
+                                                    const float4* __restrict__ grid4) {
+    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    float4 xi = pos[s];
+    const float h2 = S.h2;
+    Rows R = neighbour_rows(S, xi);
+    float sum = 0.0f;
+    for (int dk = 0; dk < 3; ++dk)
+        for (int dj = 0; dj < 3; ++dj) {
+            int b, e;
+            row_slots(range, S, R, dk, dj, b, e);
+#pragma unroll 4
+            for (int q = b; q < e; ++q) {
+                float4 xj = pos[q];
+                float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float d = fmaxf(h2 - r2, 0.0f);
+                sum += d * d * d;
+            }
+        }
+    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+}
+
+// Wavefront-cooperative density: a block owns 256 consecutive sorted
+// particles and walks them in L rounds; in a round, L adjacent lanes share one
+// particle and take every L-th candidate of its row runs.  Lanes of a
+// particle read consecutive candidates and the particles of a wavefront sit
+// in the same or adjacent cells, so one load instruction touches a few cache
+// lines instead of one run per lane.  The L partial sums meet through
+// cross-lane shuffles; the per-particle epilogue runs one thread per particle.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                         const int2* __restrict__ range, uint32_t n,
+                                                         const DevScene* __restrict__ scenes,
+                                                         const uint32_t* __restrict__ begins, int nscenes,
+                                                         const float4* __restrict__ grid4) {
+    constexpr int PR = kBlock / L;
+    __shared__ float s_sum[kBlock];
+    const uint32_t P0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+    const int c = threadIdx.x % L;
+    for (int round = 0; round < L; ++round) {
+        const int pl = round * PR + threadIdx.x / L;
+        const uint32_t s = P0 + pl;
+        float sum = 0.0f;
+        if (s < n) {
+            const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+            const float4 xi = pos[s];
+            const float h2 = S.h2;
+            Rows R = neighbour_rows(S, xi);
+            for (int dk = 0; dk < 3; ++dk)
+                for (int dj = 0; dj < 3; ++dj) {
+                    int b, e;
+                    row_slots(range, S, R, dk, dj, b, e);
+                    for (int q = b + c; q < e; q += L) {
+                        float4 xj = pos[q];
+                        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                        float r2 = dx * dx + dy * dy + dz * dz;
+                        float d = fmaxf(h2 - r2, 0.0f);
+                        sum += d * d * d;
+                    }
+                }
+        }
+#pragma unroll
+        for (int o = 1; o < L; o <<= 1) sum += __shfl_xor(sum, o, 64);
+        if (c == 0) s_sum[pl] = sum;
+    }
+    __syncthreads();
+    const uint32_t s = P0 + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const float4 xi = pos[s];
+    float rho = s_sum[threadIdx.x] * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+}
+
+// compute_particle_acceleration_ifsurf (cu:315-474, 1459-1556) + integration,
+// one thread per particle.  Per run of up to 64 candidates a cheap distance
+// test records the hits in a 64-bit mask, then the pair terms run for the
+// set bits only: roughly one candidate in six is within h, so a fused loop
+// would run the pair body for almost every candidate of a wavefront.
+__global__ __launch_bounds__(kBlock) void k_force(
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
+    const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
+    const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
+    const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status) {
+    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const float4 xi = pos[s];
+    const float4 vi = vel[s];
+    const float rhoi = vi.w;
+    const float h = S.h, h2 = S.h2;
+    const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
+    const float kr0 = S.stiff * S.rho0;
+    Rows R = neighbour_rows(S, xi);
+    PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int dk = 0; dk < 3; ++dk)
+        for (int dj = 0; dj < 3; ++dj) {
+            int b, e;
+            row_slots(range, S, R, dk, dj, b, e);
+            for (int base = b; base < e; base += 64) {
+                const int cnt = min(64, e - base);
+                uint64_t hits = 0;
+#pragma unroll 4
+                for (int t = 0; t < cnt; ++t) {
+                    float4 xj = pos[base + t];
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    hits |= (uint64_t)(r2 <= h2 && r2 > 0.0f) << t;
+                }
+                while (hits) {
+                    const int t = __ffsll((unsigned long long)hits) - 1;
+                    hits &= hits - 1;
+                    const int q = base + t;
+                    pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[q], vel[q]);
+                }
+            }
+        }
+    force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
+                 status);
+}
+
+// Wavefront-cooperative force.  Per round, L adjacent lanes share a particle:
+// they split its candidates, ballot the hits and append them to the
+// particle's hit list in LDS (one wavefront-wide vote per candidate batch,
+// no atomics), then split the list for the pair terms, so the expensive body
+// runs on dense hits only.  The ten partial sums meet through shuffles and
+// the epilogue runs one thread per particle.  A particle with more than CAP
+// hits (strong compression) is finished by its first lane with the direct
+// loop.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_force_coop(
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
+    const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
+    const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
+    const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status) {
+    static_assert(L >= 2 && L <= 16 && (L & (L - 1)) == 0, "L must be 2, 4, 8 or 16");
+    constexpr int PR = kBlock / L;
+    constexpr int CAP = L == 2 ? 64 : 96;
+    __shared__ int s_list[PR][CAP];
+    __shared__ float s_acc[10][kBlock];
+    const uint32_t P0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
+    const int lane = threadIdx.x & 63;
+    const int c = threadIdx.x % L;
+    const int gshift = (lane / L) * L;              // first lane of the group
+    const uint32_t below = (1u << c) - 1u;
+    for (int round = 0; round < L; ++round) {
+        const int gp = threadIdx.x / L;
+        const int pl = round * PR + gp;
+        const uint32_t s = P0 + pl;
+        const bool valid = s < n;
+        const DevScene& S = scenes[scene_of(begins, nscenes, valid ? s : 0)];
+        const float4 xi = valid ? pos[s] : make_float4(0, 0, 0, 0);
+        const float4 vi = valid ? vel[s] : make_float4(0, 0, 0, 1);
+        const float h = S.h, h2 = S.h2;
+        const float rhoi = vi.w;
+        const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;
+        const float kr0 = S.stiff * S.rho0;
+        Rows R = neighbour_rows(S, xi);
+        int cnt = 0;
+        // phase 1: distance tests, hits compacted into the particle's list
+        for (int dk = 0; dk < 3; ++dk)
+            for (int dj = 0; dj < 3; ++dj) {
+                int b = 0, e = 0;
+                if (valid) row_slots(range, S, R, dk, dj, b, e);
+                for (int q = b + c; __any(q < e); q += L) {
+                    bool hit = false;
+                    if (q < e) {
+                        float4 xj = pos[q];
+                        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                        float r2 = dx * dx + dy * dy + dz * dz;
+                        hit = r2 <= h2 && r2 > 0.0f;
+                    }
+                    const uint64_t vote = __ballot(hit);
+                    const uint32_t gbits = (uint32_t)(vote >> gshift) & ((1u << L) - 1u);
+                    if (hit) {
+                        int idx = cnt + __popc(gbits & below);
+                        if (idx < CAP) s_list[gp][idx] = q;
+                    }
+                    cnt += __popc(gbits);
+                }
+            }
+        __syncthreads();
+        // phase 2: pair terms over the hit list, split across the lanes
+        PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int m = min(cnt, CAP);
+        for (int i = c; i < m; i += L) {
+            const int q = s_list[gp][i];
+            pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[q], vel[q]);
+        }
+        if (cnt > CAP && c == 0) {
+            // overflow: redo this particle directly
+            a = PairSums{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int dk = 0; dk < 3; ++dk)
+                for (int dj = 0; dj < 3; ++dj) {
+                    int b, e;
+                    row_slots(range, S, R, dk, dj, b, e);
+                    for (int q = b; q < e; ++q) {
+                        float4 xj = pos[q];
+                        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                        float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= h2 && r2 > 0.0f) pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
+                    }
+                }
+        }
+        // shuffles run on every lane (uniform control flow); a particle that
+        // overflowed keeps its first lane's direct sums
+        float* av = &a.fpx;
+#pragma unroll
+        for (int t = 0; t < 10; ++t) {
+            float v = av[t];
+#pragma unroll
+            for (int o = 1; o < L; o <<= 1) v += __shfl_xor(v, o, 64);
+            av[t] = cnt <= CAP ? v : av[t];
+        }
+        if (c == 0) {
+#pragma unroll
+            for (int t = 0; t < 10; ++t) s_acc[t][pl] = av[t];
+        }
+        __syncthreads();
+    }
+    const uint32_t s = P0 + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    PairSums a;
+    float* av = &a.fpx;
+#pragma unroll
+    for (int t = 0; t < 10; ++t) av[t] = s_acc[t][threadIdx.x];
+    force_finish(S, s, pos[s], vel[s], a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out,
+                 key_out, status);
+}
+
+// Hit-mask pipeline.  The density pass already evaluates the distance of
+// every candidate of the nine row runs; it records, per particle and row,
+// which of the first 64 candidates are force neighbours (0 < r^2 <= h^2) in a
+// 64-bit mask (masks[row * n + slot], coalesced).  The force pass then skips
+// the distance tests: it expands the masks into a per-thread hit list in LDS
+// and runs the pair terms over the list, so a wavefront's trip count is the
+// largest neighbour count of its lanes rather than the sum over rows of the
+// largest per-row count.  Rows longer than 64 candidates (strong compression)
+// are handled by the force pass directly.
+__global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                          const int2* __restrict__ range, uint32_t n,
+                                                          const DevScene* __restrict__ scenes,
+                                                          const uint32_t* __restrict__ begins, int nscenes,
+                                                          const float4* __restrict__ grid4,
+                                                          uint64_t* __restrict__ masks) {
+    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    float4 xi = pos[s];
+    const float h2 = S.h2;
+    Rows R = neighbour_rows(S, xi);
+    float sum = 0.0f;
+#pragma unroll
+    for (int row = 0; row < 9; ++row) {
+        int b, e;
+        row_slots(range, S, R, row / 3, row % 3, b, e);
+        const int cnt = e - b;
+        const int c64 = min(cnt, 64);
+        uint64_t m = 0;
+#pragma unroll 4
+        for (int t = 0; t < c64; ++t) {
+            float4 xj = pos[b + t];
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float d = fmaxf(h2 - r2, 0.0f);
+            sum += d * d * d;
+            m |= (uint64_t)(r2 <= h2 && r2 > 0.0f) << t;
+        }
+        for (int t = 64; t < cnt; ++t) {
+            float4 xj = pos[b + t];
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float d = fmaxf(h2 - r2, 0.0f);
+            sum += d * d * d;
+        }
+        masks[(size_t)row * n + s] = m;
+    }
+    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+}
+
+// The hit list holds 16-bit entries (row << 6 | position in the row's run)
+// and the nine run starts per thread, so that four blocks fit the LDS of a
+// CU; the pair loop fetches the next neighbour's position and velocity
+// before it evaluates the current one, so the gathers overlap the
+// arithmetic.
+template <int CAP>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_masks(
+    const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
+    const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
+    const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
+    const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, const uint64_t* __restrict__ masks) {
+    // one column per thread: no barrier needed, consecutive threads hit
+    // consecutive banks
+    __shared__ uint16_t s_list[CAP][kBlock];
+    __shared__ int s_base[9][kBlock];
+    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const float4 xi = pos[s];
+    const float4 vi = vel[s];
+    const float rhoi = vi.w;
+    const float h = S.h, h2 = S.h2;
+    const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
+    const float kr0 = S.stiff * S.rho0;
+    Rows R = neighbour_rows(S, xi);
+    PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt = 0;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int row = 0; row < 9; ++row) {
+        int b, e;
+        row_slots(range, S, R, row / 3, row % 3, b, e);
+        s_base[row][tid] = b;
+        if (e - b <= 64) {
+            uint64_t m = masks[(size_t)row * n + s];
+            while (m) {
+                const int t = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                if (cnt < CAP) s_list[cnt][tid] = (uint16_t)(row << 6 | t);
+                else pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
+                ++cnt;
+            }
+        } else {
+            for (int q = b; q < e; ++q) {
+                float4 xj = pos[q];
+                float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= h2 && r2 > 0.0f) pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
+            }
+        }
+    }
+    const int m = min(cnt, CAP);
+    if (m > 0) {
+        int ent = s_list[0][tid];
+        int q = s_base[ent >> 6][tid] + (ent & 63);
+        float4 xj = pos[q], vj = vel[q];
+        for (int k = 1; k < m; ++k) {
+            ent = s_list[k][tid];
+            q = s_base[ent >> 6][tid] + (ent & 63);
+            const float4 xn = pos[q], vn = vel[q];
+            pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vj);
+            xj = xn;
+            vj = vn;
+        }
+        pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vj);
+    }
+    force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
+                 status);
+}
+
 }  // namespace
 
 void launch_hash(hipStream_t st, const StepBuffers& b) {
@@ -277,17 +623,46 @@ void launch_reorder(hipStream_t st, const StepBuffers& b) {
                        b.slot_of_oid);
 }
 
-void launch_density(hipStream_t st, const StepBuffers& b, int block) {
-    (void)block;
-    hipLaunchKernelGGL(k_density, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.pos_b, b.vel_b, b.range,
-                       b.n, b.scenes, b.begins, b.nscenes, b.grid4);
+void launch_density(hipStream_t st, const StepBuffers& b, int lanes) {
+    dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
+    if (b.masks) {
+        hipLaunchKernelGGL(k_density_masks, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
+                           b.nscenes, b.grid4, b.masks);
+        return;
+    }
+    switch (lanes) {
+    case 2: hipLaunchKernelGGL(k_density_coop<2>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    case 4: hipLaunchKernelGGL(k_density_coop<4>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    case 8: hipLaunchKernelGGL(k_density_coop<8>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    case 16: hipLaunchKernelGGL(k_density_coop<16>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    default: hipLaunchKernelGGL(k_density, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    }
 }
 
-void launch_force(hipStream_t st, const StepBuffers& b, int block) {
-    (void)block;
-    hipLaunchKernelGGL(k_force, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.pos_b, b.vel_b, b.range,
-                       b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc, b.pos_a,
-                       b.vel_a, b.oid_a, b.state_a, b.key, b.status);
+#define SPH_FORCE_ARGS                                                                                              \
+    b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
+        b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status
+
+void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
+    dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
+    if (b.masks) {
+        // lanes selects the LDS hit-list capacity of the hit-mask variant
+        switch (lanes) {
+        case 32: hipLaunchKernelGGL(k_force_masks<32>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
+        case 64: hipLaunchKernelGGL(k_force_masks<64>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
+        default: hipLaunchKernelGGL(k_force_masks<48>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
+        }
+        return;
+    }
+    switch (lanes) {
+    case 2: hipLaunchKernelGGL(k_force_coop<2>, g, t, 0, st, SPH_FORCE_ARGS); break;
+    case 4: hipLaunchKernelGGL(k_force_coop<4>, g, t, 0, st, SPH_FORCE_ARGS); break;
+    case 8: hipLaunchKernelGGL(k_force_coop<8>, g, t, 0, st, SPH_FORCE_ARGS); break;
+    case 16: hipLaunchKernelGGL(k_force_coop<16>, g, t, 0, st, SPH_FORCE_ARGS); break;
+    default: hipLaunchKernelGGL(k_force, g, t, 0, st, SPH_FORCE_ARGS); break;
+    }
 }
+
+#undef SPH_FORCE_ARGS
 
 }  // namespace sph

@@ -103,11 +103,10 @@ static bool boundary_valid(const sph_boundary& B, std::string& why) {
 
 Engine::~Engine() {
     if (ts) twoscale_destroy(*this);
-    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
-                    perm, slot_of_oid, state_a, state_b, range, status, n_dev};
+                    perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     sorter.release();
@@ -286,6 +285,7 @@ StepBuffers Engine::buffers() const {
     b.slot_of_oid = ts ? slot_of_oid : nullptr;
     b.range = range;
     b.status = status;
+    b.masks = tuning.hit_masks ? masks : nullptr;
     return b;
 }
 
@@ -340,14 +340,15 @@ void Engine::enqueue_step(bool two_scale) {
     mark(kStageCells);
     launch_reorder(stream, b);
     mark(kStageReorder);
-    launch_density(stream, b, 256);
+    launch_density(stream, b, tuning.density_lanes);
     mark(kStageDensity);
     if (two_scale) {
         twoscale_enqueue(*this);
         b.state_b = twoscale_final_state(*this);
     }
-    launch_force(stream, b, 256);
+    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes);
     mark(kStageForce);
+    last_two_scale = two_scale;
     keys_valid = true;
     cells_valid = true;
 }
@@ -428,12 +429,108 @@ sph_status Engine::upload_params(int s, const sph_params& p) {
     if (hipMemcpyAsync(d_scenes + s, &hscenes[s], sizeof(DevScene), hipMemcpyHostToDevice, stream) != hipSuccess)
         return fail(SPH_DEVICE_ERROR, "parameter upload failed");
     if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "parameter upload sync");
-    if (graph_exec) {
-        (void)hipGraphExecDestroy(graph_exec);
-        (void)hipGraphDestroy(graph);
-        graph_exec = nullptr;
-        graph = nullptr;
+    drop_graph();
+    return SPH_OK;
+}
+
+void Engine::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+static bool lanes_ok(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16; }
+static bool list_ok(int c) { return c == 32 || c == 48 || c == 64; }
+
+sph_status Engine::set_tuning(const sph_tuning& t) {
+    if (!lanes_ok(t.density_lanes) || !lanes_ok(t.force_lanes))
+        return fail(SPH_INVALID_ARG, "lanes per particle must be 1, 2, 4, 8 or 16");
+    if (t.hit_masks != 0 && t.hit_masks != 1) return fail(SPH_INVALID_ARG, "hit_masks must be 0 or 1");
+    if (!list_ok(t.hit_list)) return fail(SPH_INVALID_ARG, "hit_list must be 32, 48 or 64");
+    if (t.hit_masks && !masks) {
+        sph_status st = alloc(&masks, (size_t)9 * N);
+        if (st != SPH_OK) return st;
+        bytes += sizeof(uint64_t) * (size_t)9 * N;
     }
+    if (t.density_lanes != tuning.density_lanes || t.force_lanes != tuning.force_lanes ||
+        t.hit_masks != tuning.hit_masks || t.hit_list != tuning.hit_list)
+        drop_graph();
+    tuning = t;
+    return SPH_OK;
+}
+
+// Times the variants of the density and force kernels on the current sorted
+// state (the inputs of the last step): every lanes-per-particle variant of
+// each, and the hit-mask pair with every hit-list capacity.  Keeps the
+// fastest density + force combination and re-runs both with it, so the state
+// is what the last step would have produced with it.  Both kernels only read
+// the sorted buffers, so repeating them is safe.
+sph_status Engine::autotune(int reps, sph_tuning* out) {
+    if (reps < 1) return fail(SPH_INVALID_ARG, "reps must be >= 1");
+    if (!keys_valid) return fail(SPH_INVALID_ARG, "autotune needs the state of one step: advance once first");
+    if (timing) {
+        sph_status st = resolve_timing();
+        if (st != SPH_OK) return st;
+    }
+    if (!masks) {
+        sph_status st = alloc(&masks, (size_t)9 * N);
+        if (st != SPH_OK) return st;
+        bytes += sizeof(uint64_t) * (size_t)9 * N;
+    }
+    StepBuffers plain = buffers();
+    plain.masks = nullptr;
+    if (last_two_scale && ts) plain.state_b = twoscale_final_state(*this);
+    StepBuffers withm = plain;
+    withm.masks = masks;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(SPH_DEVICE_ERROR, "event creation failed");
+    }
+    bool ok = true;
+    auto time_it = [&](auto&& run) -> float {
+        run();  // warm-up
+        (void)hipEventRecord(e0, stream);
+        for (int r = 0; r < reps; ++r) run();
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            ok = false;
+            return 1e30f;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return ms;
+    };
+    static const int cand[] = {1, 2, 4, 8, 16};
+    static const int caps[] = {32, 48, 64};
+    float best_d = 1e30f, best_f = 1e30f, best_fm = 1e30f;
+    sph_tuning pick = tuning;
+    for (int L : cand) {
+        float ms = time_it([&] { launch_density(stream, plain, L); });
+        if (ms < best_d) { best_d = ms; pick.density_lanes = L; }
+    }
+    for (int L : cand) {
+        float ms = time_it([&] { launch_force(stream, plain, L); });
+        if (ms < best_f) { best_f = ms; pick.force_lanes = L; }
+    }
+    float t_dm = time_it([&] { launch_density(stream, withm, 1); });
+    for (int c : caps) {
+        float ms = time_it([&] { launch_force(stream, withm, c); });
+        if (ms < best_fm) { best_fm = ms; pick.hit_list = c; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!ok) return fail(SPH_DEVICE_ERROR, "autotune: kernel failed");
+    pick.hit_masks = (t_dm + best_fm < best_d + best_f) ? 1 : 0;
+    sph_status st = set_tuning(pick);
+    if (st != SPH_OK) return st;
+    StepBuffers& b = tuning.hit_masks ? withm : plain;
+    launch_density(stream, b, tuning.density_lanes);
+    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes);
+    st = check_launch("autotune");
+    if (st != SPH_OK) return st;
+    if (out) *out = tuning;
     return SPH_OK;
 }
 
@@ -697,6 +794,20 @@ sph_status sph_time_steps(sph_handle* h, int32_t n, int32_t two_scale, float* ms
     return h ? time_engine(h->e, n, two_scale != 0, ms) : SPH_INVALID_ARG;
 }
 
+sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t) {
+    return h && t ? h->e.set_tuning(*t) : SPH_INVALID_ARG;
+}
+
+sph_status sph_get_tuning(const sph_handle* h, sph_tuning* out) {
+    if (!h || !out) return SPH_INVALID_ARG;
+    *out = h->e.tuning;
+    return SPH_OK;
+}
+
+sph_status sph_autotune(sph_handle* h, int32_t reps, sph_tuning* out) {
+    return h ? h->e.autotune(reps, out) : SPH_INVALID_ARG;
+}
+
 // ---- batched scenes ----
 
 sph_status sph_batch_create(int32_t n_scenes, const sph_params* params, const sph_scene* scenes, sph_batch** out) {
@@ -749,6 +860,20 @@ sph_status sph_batch_use_graph(sph_batch* b, int32_t enable) {
     if (!b) return SPH_INVALID_ARG;
     b->e.use_graph = enable != 0;
     return SPH_OK;
+}
+
+sph_status sph_batch_set_tuning(sph_batch* b, const sph_tuning* t) {
+    return b && t ? b->e.set_tuning(*t) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_get_tuning(const sph_batch* b, sph_tuning* out) {
+    if (!b || !out) return SPH_INVALID_ARG;
+    *out = b->e.tuning;
+    return SPH_OK;
+}
+
+sph_status sph_batch_autotune(sph_batch* b, int32_t reps, sph_tuning* out) {
+    return b ? b->e.autotune(reps, out) : SPH_INVALID_ARG;
 }
 
 uint64_t sph_batch_num_particles(const sph_batch* b) { return b ? b->e.N : 0; }

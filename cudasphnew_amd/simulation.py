@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import (Field, SphCounts, SphParams, SPH_MAX_STAGES, as_fptr, check, lib)
+from ._native import (Field, SphCounts, SphParams, SphTuning, SPH_MAX_STAGES, as_fptr, check, lib)
 from .scenes import Scene, make_scene
 
 _FLOAT3 = {Field.POSITION, Field.VELOCITY, Field.ACCEL, Field.SUB_POSITION, Field.SUB_VELOCITY,
@@ -226,6 +226,26 @@ class ParticleSimulation:
         check(lib.sph_time_steps(self._need(), int(n), 1 if two_scale else 0, C.byref(ms)), self._h)
         return float(ms.value)
 
+    # -- kernel tuning (variants of the density / force passes) ---------------
+    def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
+                   hit_list: int = 48) -> None:
+        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list))
+        check(lib.sph_set_tuning(self._need(), C.byref(t)), self._h)
+
+    def get_tuning(self) -> dict:
+        t = SphTuning()
+        check(lib.sph_get_tuning(self._need(), C.byref(t)), self._h)
+        return _tuning_dict(t)
+
+    def autotune(self, reps: int = 5) -> dict:
+        t = SphTuning()
+        check(lib.sph_autotune(self._need(), int(reps), C.byref(t)), self._h)
+        return _tuning_dict(t)
+
+
+def _tuning_dict(t: SphTuning) -> dict:
+    return {f: int(getattr(t, f)) for f, _ in SphTuning._fields_}
+
 
 class Batch:
     """Many independent scenes advanced together (sph_batch)."""
@@ -277,3 +297,27 @@ class Batch:
         ms = C.c_float()
         check(lib.sph_batch_time_steps(self._b, int(n), C.byref(ms)), batch=self._b)
         return float(ms.value)
+
+    def enable_timing(self, on: bool = True) -> None:
+        check(lib.sph_batch_enable_timing(self._b, 1 if on else 0), batch=self._b)
+
+    def stage_times(self) -> dict:
+        ms = (C.c_float * SPH_MAX_STAGES)()
+        cnt = C.c_int32()
+        check(lib.sph_batch_stage_times(self._b, ms, C.byref(cnt)), batch=self._b)
+        return {lib.sph_stage_name(i).decode(): float(ms[i]) for i in range(cnt.value)}
+
+    def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
+                   hit_list: int = 48) -> None:
+        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list))
+        check(lib.sph_batch_set_tuning(self._b, C.byref(t)), batch=self._b)
+
+    def get_tuning(self) -> dict:
+        t = SphTuning()
+        check(lib.sph_batch_get_tuning(self._b, C.byref(t)), batch=self._b)
+        return _tuning_dict(t)
+
+    def autotune(self, reps: int = 5) -> dict:
+        t = SphTuning()
+        check(lib.sph_batch_autotune(self._b, int(reps), C.byref(t)), batch=self._b)
+        return _tuning_dict(t)

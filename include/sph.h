@@ -239,6 +239,31 @@ const char* sph_stage_name(int32_t stage);
 sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
                           float* ms);
 
+/* ---- kernel tuning -----------------------------------------------------
+ * Variants of the density and force passes; results agree to FP32 summation
+ * order.  density_lanes / force_lanes: lanes of a wavefront that share one
+ * particle in the neighbour loops (1 = one thread per particle; 2, 4, 8, 16 =
+ * cooperative variants).  hit_masks = 1: the density pass records the force
+ * neighbours of each particle as per-row bit masks (72 bytes per particle of
+ * extra device memory) and the force pass runs the pair terms from them
+ * through a per-thread hit list of hit_list entries (32, 48 or 64) in LDS;
+ * the lanes fields are then unused.  No reference counterpart: the reference
+ * fixes one thread per particle and a block size of 256
+ * (particle_simulation.cu:1385-1556); these generalise its launch
+ * configuration. */
+typedef struct sph_tuning {
+    int32_t density_lanes;
+    int32_t force_lanes;
+    int32_t hit_masks;
+    int32_t hit_list;
+} sph_tuning;
+sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t);
+sph_status sph_get_tuning(const sph_handle* h, sph_tuning* out);
+/* Times every variant on the current state (needs one step first; each
+ * kernel reps times), keeps the fastest, re-runs the last step's density and
+ * force with it; out may be NULL. */
+sph_status sph_autotune(sph_handle* h, int32_t reps, sph_tuning* out);
+
 /* ---- batched scenes ----------------------------------------------------
  * Many independent scenes advanced together: one launch per stage over all
  * scenes, scenes kept in disjoint key ranges of one segmented sort.  The
@@ -260,6 +285,10 @@ sph_status sph_batch_enable_timing(sph_batch* b, int32_t enable);
 sph_status sph_batch_stage_times(sph_batch* b, float* ms, int32_t* count);
 /* hipGraph capture of one batched step (0 = launch kernels directly). */
 sph_status sph_batch_use_graph(sph_batch* b, int32_t enable);
+/* Kernel tuning of a batch, as sph_set_tuning/sph_get_tuning/sph_autotune. */
+sph_status sph_batch_set_tuning(sph_batch* b, const sph_tuning* t);
+sph_status sph_batch_get_tuning(const sph_batch* b, sph_tuning* out);
+sph_status sph_batch_autotune(sph_batch* b, int32_t reps, sph_tuning* out);
 uint64_t sph_batch_num_particles(const sph_batch* b);
 const char* sph_batch_last_error_message(const sph_batch* b);
 

@@ -288,3 +288,121 @@ def test_stage_timing(dam16k):
     assert t["density"] > 0 and t["force"] > 0 and t["sort"] > 0
     ms = sim.time_steps(10)
     assert ms > 0
+
+
+TUNINGS = [
+    dict(density_lanes=1, force_lanes=1),
+    dict(density_lanes=2, force_lanes=2),
+    dict(density_lanes=4, force_lanes=4),
+    dict(density_lanes=8, force_lanes=8),
+    dict(density_lanes=16, force_lanes=16),
+    dict(density_lanes=4, force_lanes=16),
+    dict(hit_masks=1, hit_list=32),
+    dict(hit_masks=1, hit_list=48),
+    dict(hit_masks=1, hit_list=64),
+]
+
+
+def _tid(t):
+    return "-".join(f"{k}{v}" for k, v in t.items())
+
+
+@pytest.mark.parametrize("tuning", TUNINGS, ids=_tid)
+def test_kernel_variants_one_step_parity(dam16k, tuning):
+    """Every variant of the density and force passes meets the one-step
+    bars."""
+    sim = _sim(dam16k)
+    sim.set_tuning(**tuning)
+    got = sim.get_tuning()
+    assert all(got[k] == v for k, v in tuning.items())
+    sim.Advance(1)
+    sim.sync()
+    _check_sort(dam16k.params, dam16k.positions, sim)
+    orc = O.OracleSim(dam16k, real=0, threads=THREADS)
+    orc.step(1)
+    rho_o = orc.read(Field.DENSITY)
+    assert np.max(np.abs(sim.read(Field.DENSITY) - rho_o) / rho_o) <= 1e-5
+    _check_accel(sim.read(Field.ACCEL), orc)
+    assert np.max(np.abs(sim.positions() - orc.read(Field.POSITION))) <= 1e-6
+
+
+@pytest.mark.parametrize("tuning", [TUNINGS[1], TUNINGS[2], TUNINGS[4], TUNINGS[6], TUNINGS[8]], ids=_tid)
+def test_kernel_variants_dense_cluster(dam16k, tuning):
+    """Hundreds of neighbours per particle: the cooperative force kernel's
+    per-particle hit list overflows (those particles take the direct loop),
+    the hit-mask variant's rows exceed 64 candidates and its LDS hit list
+    overflows."""
+    rng = np.random.default_rng(7)
+    pos = (0.3 + 0.03 * rng.random((600, 3))).astype(np.float32)
+    sc = dam16k.with_positions(pos)
+    ref = _sim(sc)
+    ref.Advance(1)
+    ref.sync()
+    sim = _sim(sc)
+    sim.set_tuning(**tuning)
+    sim.Advance(1)
+    sim.sync()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1)
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
+    a, a1 = sim.read(Field.ACCEL), ref.read(Field.ACCEL)
+    scale = np.maximum(np.linalg.norm(a1, axis=1), G)[:, None]
+    assert (np.abs(a - a1) / scale).max() <= 1e-4
+
+
+def test_hit_masks_over_many_steps(dam16k):
+    """The hit-mask pipeline stays in parity over 200 steps."""
+    sim = _sim(dam16k)
+    sim.set_tuning(hit_masks=1, hit_list=32)
+    orc = O.OracleSim(dam16k, real=0, threads=THREADS)
+    sim.Advance(200)
+    orc.step(200)
+    sim.sync()
+    rho, rho_o = sim.read(Field.DENSITY), orc.read(Field.DENSITY)
+    assert abs(rho.mean() - rho_o.mean()) <= 1e-4 * rho_o.mean()
+    m = dam16k.params.particle_mass
+    ke, ke_o = kinetic_energy(m, sim.read(Field.VELOCITY)), kinetic_energy(m, orc.read(Field.VELOCITY))
+    assert abs(ke - ke_o) <= 0.01 * ke_o
+
+
+def test_autotune_keeps_parity(dam16k):
+    sim = _sim(dam16k)
+    with pytest.raises(S.SphError):
+        sim.autotune()                       # needs one step first
+    sim.Advance(1)
+    t = sim.autotune(reps=2)
+    assert t["density_lanes"] in (1, 2, 4, 8, 16) and t["force_lanes"] in (1, 2, 4, 8, 16)
+    assert t["hit_masks"] in (0, 1) and t["hit_list"] in (32, 48, 64)
+    assert sim.get_tuning() == t
+    sim.sync()
+    orc = O.OracleSim(dam16k, real=0, threads=THREADS)
+    orc.step(1)
+    rho_o = orc.read(Field.DENSITY)
+    assert np.max(np.abs(sim.read(Field.DENSITY) - rho_o) / rho_o) <= 1e-5
+    _check_accel(sim.read(Field.ACCEL), orc)
+    sim.Advance(4)
+    orc.step(4)
+    sim.sync()
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-4
+    with pytest.raises(S.SphError):
+        sim.set_tuning(density_lanes=3)
+    with pytest.raises(S.SphError):
+        sim.set_tuning(hit_masks=1, hit_list=40)
+
+
+@pytest.mark.parametrize("tuning", [TUNINGS[3], TUNINGS[7]], ids=_tid)
+def test_batch_tuning_matches_default(tuning):
+    scenes = [S.make_scene("batch", index=i, seed=1) for i in (3, 11)]
+    a, b = S.Batch(scenes), S.Batch(scenes)
+    b.set_tuning(**tuning)
+    a.step(1)
+    b.step(1)
+    a.sync()
+    b.sync()
+    for i in range(len(scenes)):
+        assert np.array_equal(a.read(i, Field.SORTED_IDS), b.read(i, Field.SORTED_IDS))
+        r, rb = a.read(i, Field.DENSITY), b.read(i, Field.DENSITY)
+        assert np.max(np.abs(rb / r - 1)) <= 1e-5
+        x, xb = a.read(i, Field.POSITION), b.read(i, Field.POSITION)
+        assert np.max(np.abs(xb - x)) <= 1e-6
+    assert b.autotune(reps=1)["density_lanes"] in (1, 2, 4, 8, 16)
