@@ -9,12 +9,13 @@ from ._native import (Config, Field, SceneDesc, SolidKind, SphBoundary, SphCount
 from .scenes import (Scene, boundary_of, config_desc, example01_boundary, make_scene, params_of,
                      particles_of, scene_from_desc, scene_sdf)
 from .simulation import Batch, ParticleSimulation
+from .neural import NeuralSDF, boundary_box, sdf_samples
 
 __all__ = [
-    "Batch", "Config", "Field", "ParticleSimulation", "Scene", "SceneDesc", "SolidKind", "SphBoundary",
+    "Batch", "Config", "Field", "NeuralSDF", "ParticleSimulation", "Scene", "SceneDesc", "SolidKind", "SphBoundary",
     "SphCounts", "SphError", "SphParams", "SphScene", "Status", "boundary_of", "config_desc",
     "example01_boundary", "lib", "make_scene", "params_of", "particles_of", "scene_from_desc",
-    "scene_sdf",
+    "scene_sdf", "boundary_box", "sdf_samples",
 ]
 
 

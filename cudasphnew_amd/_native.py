@@ -56,6 +56,7 @@ class Field(enum.IntEnum):
     DEFAULT_IDS = 18
     SUB_SORTED_KEYS = 19
     SUB_SORTED_IDS = 20
+    WALL_SOURCE = 21
 
 
 class Config(enum.IntEnum):
@@ -146,6 +147,29 @@ class SceneDesc(C.Structure):
     ]
 
 
+SPH_MLP_NUM_PARAMS = 16641
+
+
+class SphMlpConfig(C.Structure):
+    _fields_ = [
+        ("box_lo", f3), ("box_hi", f3), ("out_scale", C.c_float), ("band", C.c_float), ("seed", C.c_uint64),
+        ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+        ("lr_min", C.c_float), ("decay_steps", C.c_int32),
+    ]
+
+
+class SphNeuralConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_float), ("band", C.c_float), ("active", C.c_float), ("brick", C.c_int32)]
+
+
+class SphNeuralReport(C.Structure):
+    _fields_ = [
+        ("cells_none", C.c_uint64), ("cells_neural", C.c_uint64), ("cells_grid", C.c_uint64),
+        ("bricks_checked", C.c_uint32), ("bricks_fallback", C.c_uint32),
+        ("max_error", C.c_float), ("mean_error", C.c_float), ("samples_in_band", C.c_uint64),
+    ]
+
+
 class SphError(RuntimeError):
     """Raised for a non-OK sph_status (the reference throws
     std::runtime_error from CUDA_SAFE_CALL, util.h:24-39)."""
@@ -209,6 +233,25 @@ SIGNATURES = [
     ("sph_scene_boundary", _st, [C.POINTER(SceneDesc), fptr, fptr, fptr]),
     ("sph_scene_sdf", _st, [C.POINTER(SceneDesc), fptr, C.c_size_t, fptr]),
     ("sph_example01_boundary", _st, [i3, f3, fptr, fptr, fptr, fptr, fptr]),
+    ("sph_scene_grid_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
+    ("sph_scene_sdf_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
+    ("sph_mlp_default_config", _st, [f3, f3, C.POINTER(SphMlpConfig)]),
+    ("sph_mlp_create", _st, [C.POINTER(SphMlpConfig), C.POINTER(_vp)]),
+    ("sph_mlp_destroy", _st, [_vp]),
+    ("sph_mlp_get_config", _st, [_vp, C.POINTER(SphMlpConfig)]),
+    ("sph_mlp_set_params", _st, [_vp, fptr, C.c_size_t]),
+    ("sph_mlp_get_params", _st, [_vp, fptr, C.c_size_t]),
+    ("sph_mlp_set_samples", _st, [_vp, fptr, C.c_size_t]),
+    ("sph_mlp_gradient", _st, [_vp, C.c_uint32, C.c_int32, fptr, fptr]),
+    ("sph_mlp_train", _st, [_vp, C.c_int32, C.c_int32, fptr]),
+    ("sph_mlp_time_train", _st, [_vp, C.c_int32, C.c_int32, fptr]),
+    ("sph_mlp_eval", _st, [_vp, fptr, C.c_size_t, fptr]),
+    ("sph_mlp_last_error_message", C.c_char_p, [_vp]),
+    ("sph_attach_neural_boundary", _st, [_vp, _vp, fptr, C.c_size_t, C.POINTER(SphNeuralConfig),
+                                         C.POINTER(SphNeuralReport)]),
+    ("sph_detach_neural_boundary", _st, [_vp]),
+    ("sph_batch_attach_neural_boundary", _st, [_vp, C.c_int32, _vp, fptr, C.c_size_t,
+                                               C.POINTER(SphNeuralConfig), C.POINTER(SphNeuralReport)]),
     ("sph_abi_version", C.c_int32, []),
     ("sph_device_count", C.c_int32, []),
 ]

@@ -49,6 +49,7 @@ SOURCES = [
     ("kernels.hip", "device"),
     ("twoscale.hip", "device"),
     ("sim.hip", "device"),
+    ("neural.hip", "device"),
 ]
 
 

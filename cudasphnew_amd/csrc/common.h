@@ -47,6 +47,20 @@ struct DevScene {
     uint32_t p_begin, p_end;              // particle range of the scene
 };
 
+// Boundary source of a cell when a learned SDF is attached: no wall term (far
+// from every surface), the network's per-particle record, or the grid.
+enum : uint8_t { kWallNone = 0, kWallNeural = 1, kWallGrid = 2 };
+
+struct NeuralRefs {
+    const uint8_t* mode;    // per global cell key; null: grid everywhere
+    const uint32_t* skey;   // sorted keys
+    const float4* rec;      // per sorted slot: (grad SDF at the sampled point, SDF)
+};
+
+__device__ __forceinline__ int wall_mode(const NeuralRefs& nr, uint32_t s) {
+    return nr.mode ? (int)nr.mode[nr.skey[s]] : (int)kWallGrid;
+}
+
 // ---------------------------------------------------------------------------
 // Cell coordinates.  These must produce bit-identical keys to the FP32
 // reference expressions; no fast-math, IEEE division.

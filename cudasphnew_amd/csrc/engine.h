@@ -28,12 +28,14 @@ enum Stage {
     kStageTwoScaleSeed,
     kStageTwoScaleSubSort,
     kStageTwoScaleSubPhysics,
+    kStageWallMlp,
     kNumStages
 };
 
 const char* stage_name(int s);
 
 struct TwoScale;  // two-scale pass state (twoscale.hip)
+struct NeuralBoundary;  // learned boundary state (neural.hip)
 
 struct Engine {
     int device = 0;
@@ -88,6 +90,7 @@ struct Engine {
 
     TwoScale* ts = nullptr;
     bool last_two_scale = false;
+    NeuralBoundary* nb = nullptr;
 
     // kernel variants of the density / force passes
     sph_tuning tuning = {1, 1, 0, 48};
@@ -121,5 +124,14 @@ sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
 sph_status twoscale_counts(Engine& e, sph_counts* out);
 size_t twoscale_bytes(const Engine& e);
 uint8_t* twoscale_final_state(Engine& e);
+
+// Learned boundary (neural.hip).
+sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, size_t nval,
+                         const sph_neural_config& cfg, sph_neural_report* rep);
+void neural_destroy(Engine& e);
+void neural_enqueue(Engine& e);
+NeuralRefs neural_refs(const Engine& e);
+size_t neural_bytes(const Engine& e);
+const uint8_t* neural_modes(const Engine& e);   // per global cell, or null
 
 }  // namespace sph

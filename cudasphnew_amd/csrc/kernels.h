@@ -35,6 +35,8 @@ struct StepBuffers {
     // per-row force-neighbour masks [9][n] written by the density pass and
     // read by the force pass; null = neighbour loops without masks
     uint64_t* masks = nullptr;
+    // learned boundary records (mode == null: grid everywhere)
+    NeuralRefs nr = {nullptr, nullptr, nullptr};
 };
 
 void launch_hash(hipStream_t st, const StepBuffers& b);

@@ -164,20 +164,36 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
                                              float4* __restrict__ acc, float4* __restrict__ pos_out,
                                              float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out,
                                              uint8_t* __restrict__ state_out, uint32_t* __restrict__ key_out,
-                                             uint32_t* __restrict__ status) {
+                                             uint32_t* __restrict__ status, const NeuralRefs& nr) {
     const float rhoi = vi.w;
     const float cpf = -S.mass * S.grad_spiky, cvf = S.mu * S.mass * S.lapl_visc;
     float Fx = cvf * a.fvx + cpf * a.fpx, Fy = cvf * a.fvy + cpf * a.fpy, Fz = cvf * a.fvz + cpf * a.fpz;
+    const int wm = wall_mode(nr, s);
     float tx, ty, tz;
     texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
-    float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
-    float distWall = -g.x;
+    // distance and wall viscosity from the grid (the wall-term switch
+    // distWall < r0 and the penalty magnitude need its exact values); a cell
+    // far from every surface takes no wall term and no lookup; with the
+    // learned source the penalty and collision normals come from the
+    // network's gradient instead of central differences of the grid
+    float distWall = S.r0, wvisc = 0.0f;
+    float4 r1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (wm != kWallNone) {
+        float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
+        distWall = -g.x;
+        wvisc = g.z;
+        if (wm == kWallNeural) r1 = nr.rec[s];   // (grad D, D) of the learned distance texture at x
+    }
     if (distWall < S.r0) {
         float coeff = rhoi / (S.dt * S.dt) * (S.r0 - distWall);
-        float3 gn = dist_grad_normalized(gridd, S, tx, ty, tz);
-        Fx -= coeff * gn.x + vi.x * g.z;
-        Fy -= coeff * gn.y + vi.y * g.z;
-        Fz -= coeff * gn.z + vi.z * g.z;
+        // normalized-coordinate gradient: learned source dD/dx times
+        // dx/du = L = dims / bscale per axis
+        float3 gn = wm == kWallNeural ? make_float3(r1.x * ((float)S.bnx / S.bscale_x), r1.y * ((float)S.bny / S.bscale_y),
+                                                    r1.z * ((float)S.bnz / S.bscale_z))
+                                      : dist_grad_normalized(gridd, S, tx, ty, tz);
+        Fx -= coeff * gn.x + vi.x * wvisc;
+        Fy -= coeff * gn.y + vi.y * wvisc;
+        Fz -= coeff * gn.z + vi.z * wvisc;
     } else {
         float cm = S.mass * S.grad_poly6;
         float gx = cm * a.cgx, gy = cm * a.cgy, gz = cm * a.cgz;
@@ -193,16 +209,24 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
     acc[s] = make_float4(ax, ay, az, 0.0f);
     float vx = vi.x + S.dt * ax, vy = vi.y + S.dt * ay, vz = vi.z + S.dt * az;
     float px = xi.x + S.dt * vx, py = xi.y + S.dt * vy, pz = xi.z + S.dt * vz;
-    if (S.collision) {
+    if (S.collision && wm != kWallNone) {
+        float D;
+        float3 gw;
         float cx, cy, cz;
         texel_coords(S, px, py, pz, cx, cy, cz);
-        float D = dist_at(gridd, S, cx, cy, cz);
+        D = dist_at(gridd, S, cx, cy, cz);
         if (D > 0.0f) {
-            float3 gw = dist_grad_normalized(gridd, S, cx, cy, cz);
-            // normalized-coordinate gradient -> world gradient: divide by L
-            gw.x *= S.bscale_x * (1.0f / (float)S.bnx);
-            gw.y *= S.bscale_y * (1.0f / (float)S.bny);
-            gw.z *= S.bscale_z * (1.0f / (float)S.bnz);
+            if (wm == kWallNeural) {
+                gw = make_float3(r1.x, r1.y, r1.z);   // learned source: its gradient at the old position
+            } else {
+                gw = dist_grad_normalized(gridd, S, cx, cy, cz);
+                // normalized-coordinate gradient -> world gradient: divide by L
+                gw.x *= S.bscale_x * (1.0f / (float)S.bnx);
+                gw.y *= S.bscale_y * (1.0f / (float)S.bny);
+                gw.z *= S.bscale_z * (1.0f / (float)S.bnz);
+            }
+        }
+        if (D > 0.0f) {
             float gl = sqrtf(gw.x * gw.x + gw.y * gw.y + gw.z * gw.z);
             if (gl > 0.0f) {
                 float nx_ = gw.x / gl, ny_ = gw.y / gl, nz_ = gw.z / gl;
@@ -228,7 +252,9 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
     if (state_out) state_out[s] = state_b[s];
 }
 
-__device__ __forceinline__ float wall_density(const DevScene& S, const float4* __restrict__ grid4, float4 xi) {
+__device__ __forceinline__ float wall_density(const DevScene& S, const float4* __restrict__ grid4, float4 xi,
+                                              const NeuralRefs& nr, uint32_t s) {
+    if (wall_mode(nr, s) == kWallNone) return 0.0f;
     float tx, ty, tz;
     texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
     return lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz)).y;
@@ -241,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
                                                     const int2* __restrict__ range, uint32_t n,
                                                     const DevScene* __restrict__ scenes,
                                                     const uint32_t* __restrict__ begins, int nscenes,
-                                                    const float4* __restrict__ grid4) {
+                                                    const float4* __restrict__ grid4, NeuralRefs nr) {
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
@@ -262,7 +288,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
                 sum += d * d * d;
             }
         }
-    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
     reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
 }
 
@@ -278,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restric
                                                          const int2* __restrict__ range, uint32_t n,
                                                          const DevScene* __restrict__ scenes,
                                                          const uint32_t* __restrict__ begins, int nscenes,
-                                                         const float4* __restrict__ grid4) {
+                                                         const float4* __restrict__ grid4, NeuralRefs nr) {
     constexpr int PR = kBlock / L;
     __shared__ float s_sum[kBlock];
     const uint32_t P0 = xcd_block(blockIdx.x, gridDim.x) * kBlock;
@@ -314,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restric
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
     const float4 xi = pos[s];
-    float rho = s_sum[threadIdx.x] * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    float rho = s_sum[threadIdx.x] * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
     reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
 }
 
@@ -329,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
     const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
-    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status) {
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, NeuralRefs nr) {
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
@@ -364,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
             }
         }
     force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
-                 status);
+                 status, nr);
 }
 
 // Wavefront-cooperative force.  Per round, L adjacent lanes share a particle:
@@ -382,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
     const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
-    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status) {
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, NeuralRefs nr) {
     static_assert(L >= 2 && L <= 16 && (L & (L - 1)) == 0, "L must be 2, 4, 8 or 16");
     constexpr int PR = kBlock / L;
     constexpr int CAP = L == 2 ? 64 : 96;
@@ -476,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
 #pragma unroll
     for (int t = 0; t < 10; ++t) av[t] = s_acc[t][threadIdx.x];
     force_finish(S, s, pos[s], vel[s], a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out,
-                 key_out, status);
+                 key_out, status, nr);
 }
 
 // Hit-mask pipeline.  The density pass already evaluates the distance of
@@ -493,7 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
                                                           const DevScene* __restrict__ scenes,
                                                           const uint32_t* __restrict__ begins, int nscenes,
                                                           const float4* __restrict__ grid4,
-                                                          uint64_t* __restrict__ masks) {
+                                                          uint64_t* __restrict__ masks, NeuralRefs nr) {
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
@@ -526,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
         }
         masks[(size_t)row * n + s] = m;
     }
-    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi);
+    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
     reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
 }
 
@@ -542,7 +568,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
     const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
-    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, const uint64_t* __restrict__ masks) {
+    uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, const uint64_t* __restrict__ masks,
+    NeuralRefs nr) {
     // one column per thread: no barrier needed, consecutive threads hit
     // consecutive banks
     __shared__ uint16_t s_list[CAP][kBlock];
@@ -599,7 +626,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
         pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vj);
     }
     force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
-                 status);
+                 status, nr);
 }
 
 }  // namespace
@@ -627,30 +654,34 @@ void launch_density(hipStream_t st, const StepBuffers& b, int lanes) {
     dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
     if (b.masks) {
         hipLaunchKernelGGL(k_density_masks, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
-                           b.nscenes, b.grid4, b.masks);
+                           b.nscenes, b.grid4, b.masks, b.nr);
         return;
     }
     switch (lanes) {
-    case 2: hipLaunchKernelGGL(k_density_coop<2>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
-    case 4: hipLaunchKernelGGL(k_density_coop<4>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
-    case 8: hipLaunchKernelGGL(k_density_coop<8>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
-    case 16: hipLaunchKernelGGL(k_density_coop<16>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
-    default: hipLaunchKernelGGL(k_density, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4); break;
+    case 2: hipLaunchKernelGGL(k_density_coop<2>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 4: hipLaunchKernelGGL(k_density_coop<4>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 8: hipLaunchKernelGGL(k_density_coop<8>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 16: hipLaunchKernelGGL(k_density_coop<16>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    default: hipLaunchKernelGGL(k_density, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
     }
 }
 
 #define SPH_FORCE_ARGS                                                                                              \
     b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
-        b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status
+        b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.nr
+
+#define SPH_FORCE_ARGS_M                                                                                            \
+    b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
+        b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.masks, b.nr
 
 void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
     dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
     if (b.masks) {
         // lanes selects the LDS hit-list capacity of the hit-mask variant
         switch (lanes) {
-        case 32: hipLaunchKernelGGL(k_force_masks<32>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
-        case 64: hipLaunchKernelGGL(k_force_masks<64>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
-        default: hipLaunchKernelGGL(k_force_masks<48>, g, t, 0, st, SPH_FORCE_ARGS, b.masks); break;
+        case 32: hipLaunchKernelGGL(k_force_masks<32>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
+        case 64: hipLaunchKernelGGL(k_force_masks<64>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
+        default: hipLaunchKernelGGL(k_force_masks<48>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
         }
         return;
     }
@@ -664,5 +695,6 @@ void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
 }
 
 #undef SPH_FORCE_ARGS
+#undef SPH_FORCE_ARGS_M
 
 }  // namespace sph

@@ -566,6 +566,92 @@ sph_status sph_scene_sdf(const sph_scene_desc* d, const float* xyz, size_t n, fl
     return SPH_OK;
 }
 
+sph_status sph_scene_sdf_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
+                                 float near_width, float* xyzd) {
+    if (!d || (n && !xyzd) || !(near_frac >= 0.0f && near_frac <= 1.0f) || !(near_width > 0.0f))
+        return SPH_INVALID_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (d->boundary_dims[a] < 2) return SPH_INVALID_ARG;
+    float lo[3], ext[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = d->boundary_origin[a];
+        ext[a] = (float)(d->boundary_dims[a] - 1) * d->boundary_spacing;
+    }
+    const size_t n_near = (size_t)((double)n * near_frac + 0.5);
+    uint64_t counter = 0;
+    auto draw = [&](float p[3]) {
+        for (int a = 0; a < 3; ++a) {
+            float u = 0.5f * (unit_pm1(seed, counter++) + 1.0f);   // [0, 1)
+            p[a] = lo[a] + u * ext[a];
+        }
+    };
+    const uint64_t max_tries = 100000ull * (uint64_t)(n_near + 1);
+    uint64_t tries = 0;
+    for (size_t i = 0; i < n; ++i) {
+        float p[3], v;
+        for (;;) {
+            draw(p);
+            v = scene_sdf_point(*d, p);
+            if (i >= n_near || std::fabs(v) < near_width) break;
+            if (++tries > max_tries) return SPH_INVALID_ARG;   // band too thin to sample
+        }
+        xyzd[4 * i] = p[0];
+        xyzd[4 * i + 1] = p[1];
+        xyzd[4 * i + 2] = p[2];
+        xyzd[4 * i + 3] = v;
+    }
+    return SPH_OK;
+}
+
+// Distance texture lookup at a world point with the arithmetic of the
+// device lookup (normalized coordinate u = (x - o)/L sampled at texel
+// u*dims - 0.5, clamp addressing, linear filter as a + t (b - a)).
+static float distance_texture(const float* g, const int dims[3], const float o[3], float spacing,
+                              const float p[3]) {
+    int i0[3], i1[3];
+    float w[3];
+    for (int k = 0; k < 3; ++k) {
+        float L = (float)(dims[k] - 1) * spacing;
+        float scale = (float)((double)dims[k] / (double)L);
+        float t = (p[k] - o[k]) * scale - 0.5f;
+        t = std::fmin(std::fmax(t, -1.0f), (float)dims[k]);
+        float fl = std::floor(t);
+        w[k] = t - fl;
+        int b = (int)fl;
+        i0[k] = std::min(std::max(b, 0), dims[k] - 1);
+        i1[k] = std::min(std::max(b + 1, 0), dims[k] - 1);
+    }
+    auto at = [&](int x, int y, int z) { return g[x + (size_t)dims[0] * (y + (size_t)dims[1] * z)]; };
+    auto lerp = [](float a, float b, float t) { return a + t * (b - a); };
+    float c00 = lerp(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), w[0]);
+    float c10 = lerp(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), w[0]);
+    float c01 = lerp(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), w[0]);
+    float c11 = lerp(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), w[0]);
+    return lerp(lerp(c00, c10, w[1]), lerp(c01, c11, w[1]), w[2]);
+}
+
+sph_status sph_scene_grid_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
+                                  float near_width, float* xyzd) {
+    sph_status st = sph_scene_sdf_samples(d, seed, n, near_frac, near_width, xyzd);
+    if (st != SPH_OK || n == 0) return st;
+    const int dims[3] = {d->boundary_dims[0], d->boundary_dims[1], d->boundary_dims[2]};
+    std::vector<float> g((size_t)dims[0] * dims[1] * dims[2]);
+    GridBuilder b;
+    for (int a = 0; a < 3; ++a) {
+        b.start[a] = d->boundary_origin[a];
+        b.dims[a] = (unsigned)dims[a];
+    }
+    b.spacing = d->boundary_spacing;
+    b.rest_distance = d->rest_distance;
+    b.support = d->boundary_support;
+    b.reset();
+    b.set_distances([&](const float p[3]) { return scene_sdf_point(*d, p); });
+    b.flatten(g.data(), nullptr, nullptr);
+    for (size_t i = 0; i < n; ++i) xyzd[4 * i + 3] = distance_texture(g.data(), dims, d->boundary_origin,
+                                                                      d->boundary_spacing, xyzd + 4 * i);
+    return SPH_OK;
+}
+
 sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* density, float* viscosity) {
     if (!d) return SPH_INVALID_ARG;
     for (int a = 0; a < 3; ++a)

@@ -20,7 +20,8 @@ const char* stage_name(int s) {
     static const char* names[kNumStages] = {"hash",          "clear_cells",      "sort",
                                             "cell_ranges",   "reorder",          "density",
                                             "force",         "ts_state",         "ts_collect",
-                                            "ts_seed_project", "ts_sub_sort",    "ts_sub_physics"};
+                                            "ts_seed_project", "ts_sub_sort",    "ts_sub_physics",
+                                            "wall_mlp"};
     return (s >= 0 && s < kNumStages) ? names[s] : "";
 }
 
@@ -103,6 +104,7 @@ static bool boundary_valid(const sph_boundary& B, std::string& why) {
 
 Engine::~Engine() {
     if (ts) twoscale_destroy(*this);
+    neural_destroy(*this);
     drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
@@ -286,6 +288,7 @@ StepBuffers Engine::buffers() const {
     b.range = range;
     b.status = status;
     b.masks = tuning.hit_masks ? masks : nullptr;
+    b.nr = neural_refs(*this);
     return b;
 }
 
@@ -340,6 +343,10 @@ void Engine::enqueue_step(bool two_scale) {
     mark(kStageCells);
     launch_reorder(stream, b);
     mark(kStageReorder);
+    if (nb) {
+        neural_enqueue(*this);
+        mark(kStageWallMlp);
+    }
     launch_density(stream, b, tuning.density_lanes);
     mark(kStageDensity);
     if (two_scale) {
@@ -620,6 +627,23 @@ sph_status Engine::read_field(int s, sph_field f, void* dst, size_t nbytes) {
         }
         return SPH_OK;
     }
+    case SPH_FIELD_WALL_SOURCE: {
+        if ((st = need(n)) != SPH_OK) return st;
+        std::vector<uint32_t> keys(n);
+        if ((st = copy(keys.data(), skey + b, n * sizeof(uint32_t))) != SPH_OK) return st;
+        if ((st = load_oid(oid_b)) != SPH_OK) return st;
+        uint8_t* o = (uint8_t*)dst;
+        const uint8_t* modes = neural_modes(*this);
+        if (!modes) {
+            std::memset(o, kWallGrid, n);
+            return SPH_OK;
+        }
+        const uint32_t off = hscenes[s].cell_offset;
+        std::vector<uint8_t> m(cells_of[s]);
+        if ((st = copy(m.data(), modes + off, cells_of[s])) != SPH_OK) return st;
+        for (uint32_t i = 0; i < n; ++i) o[oid[i]] = m[keys[i] - off];
+        return SPH_OK;
+    }
     default:
         if (ts && s == 0) return twoscale_read(*this, f, dst, nbytes);
         return fail(SPH_INVALID_ARG, "field not available");
@@ -736,7 +760,7 @@ sph_status sph_get_counts(sph_handle* h, sph_counts* out) {
 size_t sph_num_cells(const sph_handle* h) { return h ? h->e.total_cells : 0; }
 
 size_t sph_device_bytes(const sph_handle* h) {
-    return h ? h->e.bytes + (h->e.ts ? sph::twoscale_bytes(h->e) : 0) : 0;
+    return h ? h->e.bytes + (h->e.ts ? sph::twoscale_bytes(h->e) : 0) + sph::neural_bytes(h->e) : 0;
 }
 
 const char* sph_last_error_message(const sph_handle* h) {
@@ -792,6 +816,19 @@ static sph_status time_engine(Engine& e, int32_t n, bool two_scale, float* ms) {
 
 sph_status sph_time_steps(sph_handle* h, int32_t n, int32_t two_scale, float* ms) {
     return h ? time_engine(h->e, n, two_scale != 0, ms) : SPH_INVALID_ARG;
+}
+
+sph_status sph_attach_neural_boundary(sph_handle* h, sph_mlp* net, const float* val_xyzd, size_t n_val,
+                                      const sph_neural_config* cfg, sph_neural_report* report) {
+    if (!h || !cfg) return SPH_INVALID_ARG;
+    return sph::neural_attach(h->e, 0, net, val_xyzd, n_val, *cfg, report);
+}
+
+sph_status sph_detach_neural_boundary(sph_handle* h) {
+    if (!h) return SPH_INVALID_ARG;
+    sph::neural_destroy(h->e);
+    h->e.drop_graph();
+    return SPH_OK;
 }
 
 sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t) {
@@ -874,6 +911,12 @@ sph_status sph_batch_get_tuning(const sph_batch* b, sph_tuning* out) {
 
 sph_status sph_batch_autotune(sph_batch* b, int32_t reps, sph_tuning* out) {
     return b ? b->e.autotune(reps, out) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_attach_neural_boundary(sph_batch* b, int32_t scene, sph_mlp* net, const float* val_xyzd,
+                                            size_t n_val, const sph_neural_config* cfg, sph_neural_report* report) {
+    if (!b || !cfg) return SPH_INVALID_ARG;
+    return sph::neural_attach(b->e, scene, net, val_xyzd, n_val, *cfg, report);
 }
 
 uint64_t sph_batch_num_particles(const sph_batch* b) { return b ? b->e.N : 0; }

@@ -26,7 +26,7 @@ def _field_array(field: Field, n: int, ncells: int, counts: SphCounts):
     nsub = counts.num_sub
     if field in (Field.CELL_START, Field.CELL_END):
         return np.empty(ncells, np.int32)
-    if field == Field.STATE:
+    if field in (Field.STATE, Field.WALL_SOURCE):
         return np.empty(n, np.uint8)
     count = {Field.SUB_POSITION: nsub, Field.SUB_VELOCITY: nsub, Field.SUB_ACCEL: nsub,
              Field.SUB_DENSITY: nsub, Field.SUB_PRESSURE: nsub, Field.SUB_IDS: nsub,
@@ -226,6 +226,15 @@ class ParticleSimulation:
         check(lib.sph_time_steps(self._need(), int(n), 1 if two_scale else 0, C.byref(ms)), self._h)
         return float(ms.value)
 
+    # -- learned boundary -------------------------------------------------------
+    def attach_neural(self, net, val=None, cfg=None) -> dict:
+        from .neural import attach
+        return attach(self, net, self.scene.desc, 0, val, cfg)
+
+    def detach_neural(self) -> None:
+        check(lib.sph_detach_neural_boundary(self._need()), self._h)
+        self._neural = None
+
     # -- kernel tuning (variants of the density / force passes) ---------------
     def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
                    hit_list: int = 48) -> None:
@@ -306,6 +315,10 @@ class Batch:
         cnt = C.c_int32()
         check(lib.sph_batch_stage_times(self._b, ms, C.byref(cnt)), batch=self._b)
         return {lib.sph_stage_name(i).decode(): float(ms[i]) for i in range(cnt.value)}
+
+    def attach_neural(self, scene: int, net, val=None, cfg=None) -> dict:
+        from .neural import attach
+        return attach(self, net, self.scenes[scene].desc, scene, val, cfg)
 
     def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
                    hit_list: int = 48) -> None:

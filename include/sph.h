@@ -157,7 +157,9 @@ typedef enum sph_field {
     SPH_FIELD_BOUNDARY_IDS = 17,/* int32 [nboundary]                      */
     SPH_FIELD_DEFAULT_IDS = 18, /* int32 [ndefault] (mParticleIdsDefault) */
     SPH_FIELD_SUB_SORTED_KEYS = 19, /* int32 [nsub] (mSubParticleHashs)   */
-    SPH_FIELD_SUB_SORTED_IDS = 20   /* int32 [nsub] sub ids, sorted       */
+    SPH_FIELD_SUB_SORTED_IDS = 20,  /* int32 [nsub] sub ids, sorted       */
+    SPH_FIELD_WALL_SOURCE = 21  /* uint8 [n]: boundary source of the last
+                                   step (0 none, 1 network, 2 grid) */
 } sph_field;
 
 /* Counts of the two-scale classes (GetNumParticles, GetNumParticlesDefault,
@@ -392,6 +394,130 @@ sph_status sph_example01_boundary(int32_t dims[3], float origin[3],
                                   float* spacing, float* rest_distance,
                                   float* distance, float* density,
                                   float* viscosity);
+
+/* Training samples of the boundary's signed distance: n points, a fraction
+ * near_frac of them within near_width of the surface (|sdf| < near_width,
+ * by rejection), the rest uniform over the boundary grid's box; each sample
+ * is (x, y, z, sdf) with sdf as sph_scene_sdf.  Seeded, deterministic. */
+sph_status sph_scene_sdf_samples(const sph_scene_desc* desc, uint64_t seed,
+                                 size_t n, float near_frac, float near_width,
+                                 float* xyzd);
+
+/* The same sample points, each labelled with the boundary grid's distance
+ * texture at the point: the field the step samples (node distances within
+ * the builder's support, clamped to +-r0, -r0 elsewhere; lookup convention
+ * of particle_simulation.cu:243-245, 3003-3007). */
+sph_status sph_scene_grid_samples(const sph_scene_desc* desc, uint64_t seed,
+                                  size_t n, float near_frac, float near_width,
+                                  float* xyzd);
+
+/* ---- learned boundary SDF ------------------------------------------------
+ * No reference counterpart: a compact stand-in for the distance texture
+ * (particle_simulation.cu:48-53, built by SphInComplexShapes.cpp:203-313).
+ * MLP over Fourier features of the normalized position
+ * u = 2 (x - lo)/(hi - lo) - 1: features [u, sin(pi f_k u_a),
+ * cos(pi f_k u_a)] for k < 16, f_k = 2^(7k/15) (99, zero-padded to 128),
+ * then 128 -> 64 -> 64 -> 64 (SiLU) -> 1 with output y.  The distance is
+ * d = band * atanh(y) (band > 0: the network learns tanh(d / band), which
+ * keeps the resolution where the boundary terms act, |d| of a few h, and
+ * saturates far from the surface) or d = out_scale * y (band = 0).
+ * Products run on BF16 MFMA with FP32 accumulation; the master parameters,
+ * biases, output layer and Adam state are FP32.
+ * Parameter vector (SPH_MLP_NUM_PARAMS floats): W1[128][64] (row = feature),
+ * b1[64], W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4. */
+#define SPH_MLP_NUM_PARAMS 16641
+typedef struct sph_mlp sph_mlp;
+typedef struct sph_mlp_config {
+    float box_lo[3], box_hi[3];   /* normalization box */
+    float out_scale;              /* metres per unit of output (band = 0) */
+    float band;                   /* tanh compression width in metres, or 0 */
+    uint64_t seed;                /* weight initialisation */
+    float lr, beta1, beta2, eps;  /* Adam */
+    float lr_min;                 /* cosine decay of lr to lr_min over */
+    int32_t decay_steps;          /* decay_steps steps (0 = constant lr) */
+} sph_mlp_config;
+/* box, out_scale = half the largest extent, band 0, seed 1, Adam (1e-3,
+ * 0.9, 0.999, 1e-8), constant lr. */
+sph_status sph_mlp_default_config(const float box_lo[3], const float box_hi[3],
+                                  sph_mlp_config* out);
+/* Xavier-uniform weights from cfg->seed, zero biases. */
+sph_status sph_mlp_create(const sph_mlp_config* cfg, sph_mlp** out);
+sph_status sph_mlp_destroy(sph_mlp* m);
+sph_status sph_mlp_get_config(const sph_mlp* m, sph_mlp_config* out);
+/* FP32 parameters (resets the Adam state and step count on set). */
+sph_status sph_mlp_set_params(sph_mlp* m, const float* params, size_t n);
+sph_status sph_mlp_get_params(sph_mlp* m, float* params, size_t n);
+/* Training set of n (x, y, z, sdf) samples, copied to the device; batches
+ * are taken in order, wrapping around. */
+sph_status sph_mlp_set_samples(sph_mlp* m, const float* xyzd, size_t n);
+/* Gradient of the loss (1/B) sum (y - t(sdf))^2, t(d) = tanh(d / band)
+ * (or d / out_scale), over the batch of
+ * samples [first, first + batch) (wrapping), without updating; batch is a
+ * multiple of 64. */
+sph_status sph_mlp_gradient(sph_mlp* m, uint32_t first, int32_t batch,
+                            float* grad, float* loss);
+/* steps Adam steps on consecutive batches; losses (optional) receives the
+ * loss of every step. */
+sph_status sph_mlp_train(sph_mlp* m, int32_t steps, int32_t batch,
+                         float* losses);
+/* Device time of `steps` training steps (HIP events). */
+sph_status sph_mlp_time_train(sph_mlp* m, int32_t steps, int32_t batch,
+                              float* ms);
+/* Distance and its gradient at n points: dgrad[4i..4i+3] = (d, dd/dx,
+ * dd/dy, dd/dz). */
+sph_status sph_mlp_eval(sph_mlp* m, const float* xyz, size_t n, float* dgrad);
+const char* sph_mlp_last_error_message(const sph_mlp* m);
+
+/* ---- learned boundary in the step -----------------------------------------
+ * The network stands in for the distance grid: every cell of the scene's
+ * cell grid takes its wall terms from
+ *   none    -- every texel the cell's particles can reach holds the grid's
+ *              far-field values (distance -r0, no wall density or
+ *              viscosity): no wall term, no lookup (exactly what the grid
+ *              gives there);
+ *   network -- per step, the network's gradient at each particle gives
+ *              the normal of the wall penalty and of the collision response
+ *              (replacing the grid's central differences, 12 trilinear
+ *              lookups); the network is trained on the distance texture
+ *              (sph_scene_grid_samples).  The distance itself, which
+ *              switches the wall term (distWall < r0, cu:422) and sets a
+ *              penalty of stiffness rho/dt^2, and the wall density and
+ *              viscosity (which depend on each wall's ghost lattice, not on
+ *              the distance alone) stay on the grid;
+ *   grid    -- the grid lookup, where validation failed: bricks of `brick`^3
+ *              cells whose validation samples (labelled as in
+ *              sph_scene_grid_samples) within `band` of the surface have an
+ *              RMS error > tolerance.
+ * The network must outlive the attachment (it is read every step).  No
+ * reference counterpart. */
+typedef struct sph_neural_config {
+    float tolerance;   /* validation bar, metres (e.g. 0.05 h) */
+    float band;        /* |sdf| below which the bar applies (e.g. 2 h) */
+    float active;      /* reserved (0) */
+    int32_t brick;     /* validation brick edge, cells */
+} sph_neural_config;
+typedef struct sph_neural_report {
+    uint64_t cells_none, cells_neural, cells_grid;
+    uint32_t bricks_checked, bricks_fallback;
+    float max_error, mean_error;   /* over validation samples within band */
+    /* a brick falls back when the RMS error of its samples within band
+     * exceeds tolerance */
+    uint64_t samples_in_band;
+} sph_neural_report;
+/* val_xyzd: n_val validation samples (x, y, z, sdf), e.g. from
+ * sph_scene_sdf_samples with another seed than the training set. */
+sph_status sph_attach_neural_boundary(sph_handle* h, sph_mlp* net,
+                                      const float* val_xyzd, size_t n_val,
+                                      const sph_neural_config* cfg,
+                                      sph_neural_report* report);
+sph_status sph_detach_neural_boundary(sph_handle* h);
+/* Learned boundary for scene `scene` of a batch, as
+ * sph_attach_neural_boundary. */
+sph_status sph_batch_attach_neural_boundary(sph_batch* b, int32_t scene,
+                                            sph_mlp* net, const float* val_xyzd,
+                                            size_t n_val,
+                                            const sph_neural_config* cfg,
+                                            sph_neural_report* report);
 
 /* ---- library info ----------------------------------------------------- */
 int32_t sph_abi_version(void);
