@@ -60,6 +60,7 @@ struct Engine {
     uint32_t* slot_of_oid = nullptr;
     uint8_t *state_a = nullptr, *state_b = nullptr;
     int2* range = nullptr;
+    uint8_t* wall_mode = nullptr;   // per global cell: kWallNone / kWallNeural / kWallGrid
     uint32_t* status = nullptr;
     uint32_t* n_dev = nullptr;
     RadixSort sorter;
@@ -130,8 +131,7 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
                          const sph_neural_config& cfg, sph_neural_report* rep);
 void neural_destroy(Engine& e);
 void neural_enqueue(Engine& e);
-NeuralRefs neural_refs(const Engine& e);
 size_t neural_bytes(const Engine& e);
-const uint8_t* neural_modes(const Engine& e);   // per global cell, or null
+const float4* neural_records(const Engine& e);   // per sorted slot, or null
 
 }  // namespace sph

@@ -43,6 +43,10 @@ void launch_hash(hipStream_t st, const StepBuffers& b);
 void launch_cells_clear(hipStream_t st, const StepBuffers& b);
 void launch_cells(hipStream_t st, const StepBuffers& b);
 void launch_reorder(hipStream_t st, const StepBuffers& b);
+// Boundary source of every cell of scene S (mode + S.cell_offset ...): none
+// where the cell's reachable texels hold only the far-field values, else
+// near_value.
+void launch_wall_modes(hipStream_t st, const DevScene& S, const float4* grid4, uint8_t near_value, uint8_t* mode);
 // lanes: lanes per particle of the neighbour loops (1 = one thread per
 // particle; 2, 4, 8, 16 = wavefront-cooperative variants).  With b.masks set
 // the hit-mask pair is launched instead and the force's lanes value selects

@@ -629,7 +629,48 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                  status, nr);
 }
 
+// Boundary source of the cells of one scene from its grid: a cell whose
+// particles can only reach texels holding the far-field values (distance
+// -r0, no wall density or viscosity) takes no wall term and no lookup --
+// exactly what the lookup would return.  The footprint is the cell's texel
+// range widened by two texels on each side (the central differences of the
+// penalty, the collision lookup after a move of less than a spacing).
+__global__ void k_wall_modes(DevScene S, const float4* __restrict__ grid4, uint8_t near_value,
+                             uint8_t* __restrict__ mode) {
+    const uint32_t ncell = (uint32_t)S.nx * S.ny * S.nz;
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int cx = (int)(c % S.nx), cy = (int)((c / S.nx) % S.ny), cz = (int)(c / ((uint32_t)S.nx * S.ny));
+    const float lo[3] = {S.ox + cx * S.cell, S.oy + cy * S.cell, S.oz + cz * S.cell};
+    const float bo[3] = {S.box_, S.boy_, S.boz_};
+    const float sc[3] = {S.bscale_x, S.bscale_y, S.bscale_z};
+    const int nd[3] = {S.bnx, S.bny, S.bnz};
+    int i0[3], i1[3];
+    for (int a = 0; a < 3; ++a) {
+        float t0 = (lo[a] - bo[a]) * sc[a] - 0.5f, t1 = (lo[a] + S.cell - bo[a]) * sc[a] - 0.5f;
+        i0[a] = max((int)floorf(t0) - 2, 0);
+        i1[a] = min((int)floorf(t1) + 3, nd[a] - 1);
+    }
+    const float4* g = grid4 + S.grid_offset;
+    bool far = true;
+    for (int k = i0[2]; k <= i1[2] && far; ++k)
+        for (int j = i0[1]; j <= i1[1] && far; ++j)
+            for (int i = i0[0]; i <= i1[0]; ++i) {
+                float4 v = g[(size_t)i + (size_t)nd[0] * ((size_t)j + (size_t)nd[1] * k)];
+                if (!(v.x == -S.r0 && v.y == 0.0f && v.z == 0.0f)) {
+                    far = false;
+                    break;
+                }
+            }
+    mode[S.cell_offset + c] = far ? kWallNone : near_value;
+}
+
 }  // namespace
+
+void launch_wall_modes(hipStream_t st, const DevScene& S, const float4* grid4, uint8_t near_value, uint8_t* mode) {
+    const uint32_t ncell = (uint32_t)S.nx * S.ny * S.nz;
+    hipLaunchKernelGGL(k_wall_modes, dim3((ncell + 255) / 256), dim3(256), 0, st, S, grid4, near_value, mode);
+}
 
 void launch_hash(hipStream_t st, const StepBuffers& b) {
     hipLaunchKernelGGL(k_hash, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.pos_a, b.key, b.n, b.scenes,

@@ -627,41 +627,6 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
     }
 }
 
-// Initial cell modes of one scene from the grid: a cell whose particles
-// can only reach texels holding the far-field values (distance -r0, no
-// wall density or viscosity) takes no wall term; the footprint is the
-// cell's texel range widened by two texels on each side (the central
-// differences of the penalty, the collision lookup after a move of less
-// than a spacing).  Everything else takes the network.
-__global__ void k_cell_modes(DevScene S, const float4* __restrict__ grid4, uint8_t* __restrict__ mode) {
-    const uint32_t ncell = (uint32_t)S.nx * S.ny * S.nz;
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
-    const int cx = (int)(c % S.nx), cy = (int)((c / S.nx) % S.ny), cz = (int)(c / ((uint32_t)S.nx * S.ny));
-    const float lo[3] = {S.ox + cx * S.cell, S.oy + cy * S.cell, S.oz + cz * S.cell};
-    const float bo[3] = {S.box_, S.boy_, S.boz_};
-    const float sc[3] = {S.bscale_x, S.bscale_y, S.bscale_z};
-    const int nd[3] = {S.bnx, S.bny, S.bnz};
-    int i0[3], i1[3];
-    for (int a = 0; a < 3; ++a) {
-        float t0 = (lo[a] - bo[a]) * sc[a] - 0.5f, t1 = (lo[a] + S.cell - bo[a]) * sc[a] - 0.5f;
-        i0[a] = max((int)floorf(t0) - 2, 0);
-        i1[a] = min((int)floorf(t1) + 3, nd[a] - 1);
-    }
-    const float4* g = grid4 + S.grid_offset;
-    bool far = true;
-    for (int k = i0[2]; k <= i1[2] && far; ++k)
-        for (int j = i0[1]; j <= i1[1] && far; ++j)
-            for (int i = i0[0]; i <= i1[0]; ++i) {
-                float4 v = g[(size_t)i + (size_t)nd[0] * ((size_t)j + (size_t)nd[1] * k)];
-                if (!(v.x == -S.r0 && v.y == 0.0f && v.z == 0.0f)) {
-                    far = false;
-                    break;
-                }
-            }
-    mode[c] = far ? kWallNone : kWallNeural;
-}
-
 // Validation: per brick of cells, the sum of squared errors of the network
 // (and their count) over the given (x, y, z, distance texture) samples
 // where the label exceeds -band (the field is the constant -r0 farther out
@@ -716,10 +681,6 @@ __global__ void k_apply_bricks(DevScene S, const float* __restrict__ sq_tight, c
     if (mode[c] == kWallNeural && k > 0 && sq_tight[b] > tol * tol * (float)k) mode[c] = kWallGrid;
 }
 
-__global__ void k_fill_u8(uint8_t* p, size_t n, uint8_t v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
 
 }  // namespace nn
 }  // namespace sph
@@ -767,7 +728,6 @@ namespace sph {
 
 // Engine-side state of the learned boundary.
 struct NeuralBoundary {
-    uint8_t* mode = nullptr;      // [total cells]
     float4* rec = nullptr;        // [N]
     nn::NetView* nets = nullptr;  // [nscenes]
     uint2* tiles = nullptr;
@@ -781,31 +741,26 @@ void neural_destroy(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb) return;
     (void)hipStreamSynchronize(e.stream);
-    void* ptrs[] = {nb->mode, nb->rec, nb->nets, nb->tiles};
+    void* ptrs[] = {nb->rec, nb->nets, nb->tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    // the cells of the scenes that had a network go back to the grid
+    for (int s = 0; s < e.nscenes; ++s)
+        if (nb->enabled[s]) launch_wall_modes(e.stream, e.hscenes[s], e.grid4, kWallGrid, e.wall_mode);
+    (void)hipStreamSynchronize(e.stream);
     delete nb;
     e.nb = nullptr;
 }
 
 size_t neural_bytes(const Engine& e) { return e.nb ? e.nb->bytes : 0; }
 
-const uint8_t* neural_modes(const Engine& e) { return e.nb ? e.nb->mode : nullptr; }
-
-NeuralRefs neural_refs(const Engine& e) {
-    NeuralRefs r = {nullptr, nullptr, nullptr};
-    if (!e.nb || !e.nb->ntiles) return r;
-    r.mode = e.nb->mode;
-    r.skey = e.skey;
-    r.rec = e.nb->rec;
-    return r;
-}
+const float4* neural_records(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->rec : nullptr; }
 
 void neural_enqueue(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb || !nb->ntiles) return;
     hipLaunchKernelGGL(nn::k_wall_mlp, dim3(nb->ntiles), dim3(nn::kThreads), 0, e.stream, nb->tiles, e.pos_b, e.skey,
-                       nb->mode, e.d_scenes, nb->nets, nb->rec);
+                       e.wall_mode, e.d_scenes, nb->nets, nb->rec);
 }
 
 static sph_status rebuild_tiles(Engine& e) {
@@ -838,17 +793,14 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         e.nb = nb;
         nb->hnets.resize(e.nscenes);
         nb->enabled.assign(e.nscenes, 0);
-        bool ok = hipMalloc(&nb->mode, e.total_cells) == hipSuccess &&
-                  hipMalloc(&nb->rec, sizeof(float4) * (size_t)e.N) == hipSuccess &&
+        bool ok = hipMalloc(&nb->rec, sizeof(float4) * (size_t)e.N) == hipSuccess &&
                   hipMalloc(&nb->nets, sizeof(nn::NetView) * e.nscenes) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
             neural_destroy(e);
             return e.fail(SPH_OUT_OF_MEMORY, "learned boundary buffers");
         }
-        nb->bytes = e.total_cells + sizeof(float4) * (size_t)e.N;
-        hipLaunchKernelGGL(nn::k_fill_u8, dim3((unsigned)((e.total_cells + 255) / 256)), dim3(256), 0, e.stream,
-                           nb->mode, (size_t)e.total_cells, (uint8_t)kWallGrid);
+        nb->bytes = sizeof(float4) * (size_t)e.N;
     }
     NeuralBoundary* nb = e.nb;
     const DevScene& S = e.hscenes[scene];
@@ -858,8 +810,8 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         return e.fail(SPH_DEVICE_ERROR, "network upload");
     // cell modes: far-field cells from the grid, the rest take the network
     const uint32_t ncell = (uint32_t)S.nx * S.ny * S.nz;
-    uint8_t* mode = nb->mode + S.cell_offset;
-    hipLaunchKernelGGL(nn::k_cell_modes, dim3((ncell + 255) / 256), dim3(256), 0, e.stream, S, e.grid4, mode);
+    uint8_t* mode = e.wall_mode + S.cell_offset;
+    launch_wall_modes(e.stream, S, e.grid4, kWallNeural, e.wall_mode);
     // validation per brick
     const int br = cfg.brick;
     const int nbx = (S.nx + br - 1) / br, nby = (S.ny + br - 1) / br, nbz = (S.nz + br - 1) / br;

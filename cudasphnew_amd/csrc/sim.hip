@@ -108,7 +108,7 @@ Engine::~Engine() {
     drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
-                    perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks};
+                    perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     sorter.release();
@@ -197,6 +197,7 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
     SPH_TRY(alloc(&d_scenes, S));
     SPH_TRY(alloc(&d_begins, S + 1));
     SPH_TRY(alloc(&grid4, nodes));
+    SPH_TRY(alloc(&wall_mode, total_cells));
     SPH_TRY(alloc(&gridd, nodes));
     SPH_TRY(alloc(&pos_a, N));
     SPH_TRY(alloc(&vel_a, N));
@@ -265,6 +266,8 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
         hipMemset(skey, 0, N * sizeof(uint32_t)) != hipSuccess ||
         hipMemset(oid_b, 0, N * sizeof(uint32_t)) != hipSuccess)
         return fail(SPH_DEVICE_ERROR, "initial clear failed");
+    // boundary source of every cell (far-field cells skip the lookups)
+    for (int s = 0; s < S; ++s) launch_wall_modes(stream, hscenes[s], grid4, kWallGrid, wall_mode);
     if (hipDeviceSynchronize() != hipSuccess) return fail(SPH_DEVICE_ERROR, "device sync after create failed");
 #undef SPH_TRY
     return SPH_OK;
@@ -288,7 +291,9 @@ StepBuffers Engine::buffers() const {
     b.range = range;
     b.status = status;
     b.masks = tuning.hit_masks ? masks : nullptr;
-    b.nr = neural_refs(*this);
+    b.nr.mode = wall_mode;
+    b.nr.skey = skey;
+    b.nr.rec = neural_records(*this);
     return b;
 }
 
@@ -633,14 +638,9 @@ sph_status Engine::read_field(int s, sph_field f, void* dst, size_t nbytes) {
         if ((st = copy(keys.data(), skey + b, n * sizeof(uint32_t))) != SPH_OK) return st;
         if ((st = load_oid(oid_b)) != SPH_OK) return st;
         uint8_t* o = (uint8_t*)dst;
-        const uint8_t* modes = neural_modes(*this);
-        if (!modes) {
-            std::memset(o, kWallGrid, n);
-            return SPH_OK;
-        }
         const uint32_t off = hscenes[s].cell_offset;
         std::vector<uint8_t> m(cells_of[s]);
-        if ((st = copy(m.data(), modes + off, cells_of[s])) != SPH_OK) return st;
+        if ((st = copy(m.data(), wall_mode + off, cells_of[s])) != SPH_OK) return st;
         for (uint32_t i = 0; i < n; ++i) o[oid[i]] = m[keys[i] - off];
         return SPH_OK;
     }
