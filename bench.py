@@ -11,6 +11,13 @@ particles, inputs resident in HBM.  K steps are timed with HIP events on the
 simulation's stream after W warm-up steps; per-stage device times come from
 events recorded around each stage inside the same timed region.
 
+Besides the headline (plain 1M), the line carries the other forms of the
+same step measured in the same run (`forms`): the 1M scene with the learned
+boundary attached, and 64 independent scenes of uneven size (log-uniform
+4,096..262,144 particles, seed 1) run as 64 separate handles, as one batch
+(one launch per stage, hipGraph-captured step) and as one batch with a
+learned boundary per scene.  `--no-forms` skips them.
+
 Prints one JSON line (stdout).  Progress goes to stderr.
 """
 from __future__ import annotations
@@ -105,6 +112,83 @@ def cpu_baseline(scene, seconds: float, threads: int):
     }
 
 
+def time_handles(sims, steps: int) -> float:
+    """Wall time per step of separate handles stepped round-robin (each on
+    its own stream), synchronised at both ends."""
+    for s in sims:
+        s.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        for s in sims:
+            s.Advance(1)
+    for s in sims:
+        s.sync()
+    return (time.perf_counter() - t) / steps * 1e3
+
+
+def bench_forms(S, a, plain_ms: float, sc) -> dict:
+    from cudasphnew_amd import neural as N
+    forms = {}
+    # 1M scene with the learned boundary
+    t0 = time.time()
+    net = N.train_for_scene(sc.desc, steps=a.train_steps)
+    sim = S.ParticleSimulation(sc).Init()
+    rep = sim.attach_neural(net)
+    sim.Advance(max(a.warmup, 1))
+    tun = sim.autotune(reps=3)
+    sim.Advance(2)
+    ms = sim.time_steps(a.steps) / a.steps
+    sim.sync()
+    forms["plain_1m_neural"] = {
+        "value": round(sc.n / (ms / 1e3), 1), "ms_per_step": round(ms, 5), "particles": sc.n,
+        "vs_plain": round(plain_ms / ms, 4), "train_steps": a.train_steps,
+        "cells": {"none": rep["cells_none"], "network": rep["cells_neural"], "grid": rep["cells_grid"]},
+        "band_mean_error_h": round(rep["mean_error"] / float(sc.desc.h), 5), "kernel_variants": tun,
+    }
+    log(f"1M + learned boundary: {ms:.4f} ms/step ({time.time() - t0:.0f}s incl. training)")
+    del sim
+    # 64 scenes
+    t0 = time.time()
+    scenes = [S.make_scene("batch", index=i, seed=1) for i in range(64)]
+    n = sum(x.n for x in scenes)
+    log(f"64 scenes, {n} particles ({time.time() - t0:.0f}s)")
+    sims = [S.ParticleSimulation(x).Init() for x in scenes]
+    for x in sims:
+        x.Advance(max(a.warmup, 1))
+    ms_h = time_handles(sims, a.batch_steps)
+    forms["handles_64"] = {"value": round(n / (ms_h / 1e3), 1), "ms_per_step": round(ms_h, 5), "particles": n,
+                           "note": "64 separate handles stepped round-robin, wall clock"}
+    log(f"64 handles: {ms_h:.4f} ms/step")
+    for x in sims:
+        x.close()
+    del sims
+    for neural in (False, True):
+        b = S.Batch(scenes)
+        if neural:
+            t1 = time.time()
+            bad = 0
+            for i, x in enumerate(scenes):
+                nt = N.train_for_scene(x.desc, steps=a.batch_train_steps, n_samples=1 << 20)
+                rep = b.attach_neural(i, nt, val=N.validation_samples(x.desc, 1 << 18))
+                bad += rep["cells_grid"]
+            log(f"64 networks trained and attached in {time.time() - t1:.0f}s ({bad} grid-fallback cells)")
+        b.use_graph(True)
+        b.step(max(a.warmup, 1))
+        tun = b.autotune(reps=2)
+        b.step(2)
+        ms_b = b.time_steps(a.batch_steps) / a.batch_steps
+        b.sync()
+        key = "batched_64_neural" if neural else "batched_64"
+        forms[key] = {"value": round(n / (ms_b / 1e3), 1), "ms_per_step": round(ms_b, 5), "particles": n,
+                      "vs_handles": round(ms_h / ms_b, 4), "vs_plain_1m": round((n / ms_b) / (1048576 / plain_ms), 4),
+                      "kernel_variants": tun, "graph": True}
+        if neural:
+            forms[key]["train_steps_per_scene"] = a.batch_train_steps
+        log(f"{key}: {ms_b:.4f} ms/step")
+        b.close()
+    return forms
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +201,10 @@ def main(argv=None) -> int:
                     help="run this many steps with no timing events and exit (for counter passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
+    ap.add_argument("--no-forms", action="store_true", help="skip the learned-boundary and batched forms")
+    ap.add_argument("--train-steps", type=int, default=20000, help="training steps of the 1M scene's network")
+    ap.add_argument("--batch-train-steps", type=int, default=2000, help="training steps per batch scene network")
+    ap.add_argument("--batch-steps", type=int, default=50, help="timed steps of the batched forms")
     ap.add_argument("--tuning", default="auto",
                     help="density/force kernel variants: 'auto' (autotune after the warm-up) or "
                          "'D,F,M,C' = density lanes, force lanes, hit masks (0/1), hit-list capacity")
@@ -179,6 +267,10 @@ def main(argv=None) -> int:
         "avg_launch_ms": round(single[dom], 5),
         "touched_cells": tc,
     }
+    forms = None
+    if not a.no_forms:
+        del sim
+        forms = bench_forms(S, a, ms / a.steps, sc)
     cpu = None
     if not a.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
@@ -203,6 +295,7 @@ def main(argv=None) -> int:
         "stage_ms": {k: round(v, 5) for k, v in stages.items() if v > 0},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -54,7 +54,9 @@ enum : uint8_t { kWallNone = 0, kWallNeural = 1, kWallGrid = 2 };
 struct NeuralRefs {
     const uint8_t* mode;    // per global cell key; null: grid everywhere
     const uint32_t* skey;   // sorted keys
-    const float4* rec;      // per sorted slot: (grad SDF at the sampled point, SDF)
+    const float4* rec;      // per sorted slot: (gradient, value) of the learned distance
+    uint8_t* need;          // per sorted slot: learned-source particle in the wall-penalty
+                            // zone (written by the density pass, read by the network pass)
 };
 
 __device__ __forceinline__ int wall_mode(const NeuralRefs& nr, uint32_t s) {

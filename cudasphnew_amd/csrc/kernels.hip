@@ -182,15 +182,17 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
         float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
         distWall = -g.x;
         wvisc = g.z;
-        if (wm == kWallNeural) r1 = nr.rec[s];   // (grad D, D) of the learned distance texture at x
+        if (wm == kWallNeural && nr.need[s]) r1 = nr.rec[s];   // (grad D, D) of the learned distance texture at x
     }
+    // the network's normal exists where the density pass asked for it
+    const bool net_normal = wm == kWallNeural && nr.need[s];
     if (distWall < S.r0) {
         float coeff = rhoi / (S.dt * S.dt) * (S.r0 - distWall);
         // normalized-coordinate gradient: learned source dD/dx times
         // dx/du = L = dims / bscale per axis
-        float3 gn = wm == kWallNeural ? make_float3(r1.x * ((float)S.bnx / S.bscale_x), r1.y * ((float)S.bny / S.bscale_y),
-                                                    r1.z * ((float)S.bnz / S.bscale_z))
-                                      : dist_grad_normalized(gridd, S, tx, ty, tz);
+        float3 gn = net_normal ? make_float3(r1.x * ((float)S.bnx / S.bscale_x), r1.y * ((float)S.bny / S.bscale_y),
+                                             r1.z * ((float)S.bnz / S.bscale_z))
+                               : dist_grad_normalized(gridd, S, tx, ty, tz);
         Fx -= coeff * gn.x + vi.x * wvisc;
         Fy -= coeff * gn.y + vi.y * wvisc;
         Fz -= coeff * gn.z + vi.z * wvisc;
@@ -216,7 +218,7 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
         texel_coords(S, px, py, pz, cx, cy, cz);
         D = dist_at(gridd, S, cx, cy, cz);
         if (D > 0.0f) {
-            if (wm == kWallNeural) {
+            if (net_normal) {
                 gw = make_float3(r1.x, r1.y, r1.z);   // learned source: its gradient at the old position
             } else {
                 gw = dist_grad_normalized(gridd, S, cx, cy, cz);
@@ -254,10 +256,15 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
 
 __device__ __forceinline__ float wall_density(const DevScene& S, const float4* __restrict__ grid4, float4 xi,
                                               const NeuralRefs& nr, uint32_t s) {
-    if (wall_mode(nr, s) == kWallNone) return 0.0f;
+    const int wm = wall_mode(nr, s);
+    if (wm == kWallNone) return 0.0f;
     float tx, ty, tz;
     texel_coords(S, xi.x, xi.y, xi.z, tx, ty, tz);
-    return lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz)).y;
+    const float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
+    // learned source: the force pass needs the network's normal exactly
+    // where the wall penalty acts (distWall = -D < r0)
+    if (wm == kWallNeural && nr.need) nr.need[s] = g.x > -S.r0;
+    return g.y;
 }
 
 // compute_particle_density_pressure (cu:196-256, 1385-1419), one thread per

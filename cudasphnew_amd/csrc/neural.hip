@@ -106,13 +106,22 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
+// sin(pi x) and cos(pi x): exact reduction to r = x - 2 rint(x/2) in
+// [-1, 1] (|x| < 2^22 here), then the hardware sine/cosine of r/2
+// revolutions (v_sin_f32 / v_cos_f32 take revolutions).
+__device__ __forceinline__ void sincospi_hw(float x, float& s, float& c) {
+    const float t = 0.5f * (x - 2.0f * rintf(0.5f * x));
+    s = __builtin_amdgcn_sinf(t);
+    c = __builtin_amdgcn_cosf(t);
+}
+
 // Feature f of normalized coordinates u.
 __device__ __forceinline__ float feature(const NetView& net, int f, const float u[3]) {
     if (f < 3) return u[f];
     if (f >= kFeat) return 0.0f;
     int i = f - 3, k = i / 6, a = (i % 6) >> 1;
     float s, c;
-    sincospif(net.freq[k] * u[a], &s, &c);
+    sincospi_hw(net.freq[k] * u[a], s, c);
     return (i & 1) ? c : s;
 }
 
@@ -129,7 +138,7 @@ __device__ __forceinline__ float feature_du(const NetView& net, int f, const flo
     int i = f - 3, k = i / 6, a = (i % 6) >> 1;
     axis = a;
     float w = net.freq[k], s, c;
-    sincospif(w * u[a], &s, &c);
+    sincospi_hw(w * u[a], s, c);
     float pw = 3.14159265358979f * w;
     return (i & 1) ? -pw * s : pw * c;
 }
@@ -164,11 +173,22 @@ __device__ __forceinline__ void input_layer(const NetView& net, const float u[3]
                                             uint16_t* sA0T) {
 #pragma unroll
     for (int ks = 0; ks < kK1 / 32; ++ks) {
+        // features f0..f0+7 (f0 = 8q) use the (sin, cos) pairs
+        // m = 4q - 2 .. 4q + 2 (pair m = 3k + a: features 3 + 2m, 4 + 2m)
+        const int q = 4 * ks + (lane >> 4), f0 = 8 * q;
+        float ps[5], pc[5];
+#pragma unroll
+        for (int t = 0; t < 5; ++t) {
+            const int m = 4 * q - 2 + t;
+            ps[t] = pc[t] = 0.0f;
+            if (m >= 0 && m < 3 * kFreqs) sincospi_hw(net.freq[m / 3] * u[m % 3], ps[t], pc[t]);
+        }
         uint16_t v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            int f = 32 * ks + 8 * (lane >> 4) + j;
-            v[j] = f2bf(feature(net, f, u));
+            const int f = f0 + j, t = (j + 1) >> 1;
+            const float g = f < 3 ? u[f < 3 ? f : 0] : (f >= kFeat ? 0.0f : ((j & 1) ? ps[t] : pc[t]));
+            v[j] = f2bf(g);
             if (sA0T) sA0T[f * kLds + 16 * w + (lane & 15)] = v[j];
         }
         uint4 pk;
@@ -592,12 +612,14 @@ __global__ __launch_bounds__(kThreads) void k_mlp_eval(const float4* __restrict_
 // ---- learned boundary in the step ------------------------------------------
 
 // Per-step distance records (gradient, value) of the particles in
-// learned-source cells, one tile of 64 sorted particles per block; tiles
-// without such a particle exit at once.
+// learned-source cells that the density pass found in the wall-penalty
+// zone, one tile of 64 sorted particles per block; tiles without such a
+// particle exit at once.
 __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__ tiles,
                                                        const float4* __restrict__ pos,
                                                        const uint32_t* __restrict__ skey,
                                                        const uint8_t* __restrict__ mode,
+                                                       const uint8_t* __restrict__ need,
                                                        const DevScene* __restrict__ scenes,
                                                        const NetView* __restrict__ nets,
                                                        float4* __restrict__ rec) {
@@ -609,7 +631,7 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
     const uint32_t s = p0 + 16 * w + (lane & 15);
     const bool valid = s < end;
     const float4 x = pos[valid ? s : p0];
-    const bool mine = valid && mode[skey[s]] == kWallNeural;
+    const bool mine = valid && need[s] && mode[skey[s]] == kWallNeural;
     if (!__syncthreads_or(mine)) return;
     const NetView net = nets[t.x];
     float u[3], y[4], gu[4][3];
@@ -618,7 +640,7 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
     if ((lane & 15) < 4) {
         const int j = lane & 15;
         const uint32_t sj = p0 + 16 * w + 4 * (lane >> 4) + j;
-        if (sj < end && mode[skey[sj]] == kWallNeural) {
+        if (sj < end && need[sj] && mode[skey[sj]] == kWallNeural) {
             float sc;
             const float d = decode(net, y[j], sc);
             rec[sj] = make_float4(sc * gu[j][0] * 2.0f * net.inv_ext[0], sc * gu[j][1] * 2.0f * net.inv_ext[1],
@@ -729,6 +751,7 @@ namespace sph {
 // Engine-side state of the learned boundary.
 struct NeuralBoundary {
     float4* rec = nullptr;        // [N]
+    uint8_t* need = nullptr;      // [N]
     nn::NetView* nets = nullptr;  // [nscenes]
     uint2* tiles = nullptr;
     uint32_t ntiles = 0;
@@ -741,7 +764,7 @@ void neural_destroy(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb) return;
     (void)hipStreamSynchronize(e.stream);
-    void* ptrs[] = {nb->rec, nb->nets, nb->tiles};
+    void* ptrs[] = {nb->rec, nb->need, nb->nets, nb->tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     // the cells of the scenes that had a network go back to the grid
@@ -756,11 +779,13 @@ size_t neural_bytes(const Engine& e) { return e.nb ? e.nb->bytes : 0; }
 
 const float4* neural_records(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->rec : nullptr; }
 
+uint8_t* neural_need(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->need : nullptr; }
+
 void neural_enqueue(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb || !nb->ntiles) return;
     hipLaunchKernelGGL(nn::k_wall_mlp, dim3(nb->ntiles), dim3(nn::kThreads), 0, e.stream, nb->tiles, e.pos_b, e.skey,
-                       e.wall_mode, e.d_scenes, nb->nets, nb->rec);
+                       e.wall_mode, nb->need, e.d_scenes, nb->nets, nb->rec);
 }
 
 static sph_status rebuild_tiles(Engine& e) {
@@ -794,13 +819,14 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         nb->hnets.resize(e.nscenes);
         nb->enabled.assign(e.nscenes, 0);
         bool ok = hipMalloc(&nb->rec, sizeof(float4) * (size_t)e.N) == hipSuccess &&
+                  hipMalloc(&nb->need, (size_t)e.N) == hipSuccess &&
                   hipMalloc(&nb->nets, sizeof(nn::NetView) * e.nscenes) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
             neural_destroy(e);
             return e.fail(SPH_OUT_OF_MEMORY, "learned boundary buffers");
         }
-        nb->bytes = sizeof(float4) * (size_t)e.N;
+        nb->bytes = (sizeof(float4) + 1) * (size_t)e.N;
     }
     NeuralBoundary* nb = e.nb;
     const DevScene& S = e.hscenes[scene];

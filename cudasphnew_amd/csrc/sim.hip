@@ -294,6 +294,7 @@ StepBuffers Engine::buffers() const {
     b.nr.mode = wall_mode;
     b.nr.skey = skey;
     b.nr.rec = neural_records(*this);
+    b.nr.need = neural_need(*this);
     return b;
 }
 
@@ -348,15 +349,15 @@ void Engine::enqueue_step(bool two_scale) {
     mark(kStageCells);
     launch_reorder(stream, b);
     mark(kStageReorder);
-    if (nb) {
-        neural_enqueue(*this);
-        mark(kStageWallMlp);
-    }
     launch_density(stream, b, tuning.density_lanes);
     mark(kStageDensity);
     if (two_scale) {
         twoscale_enqueue(*this);
         b.state_b = twoscale_final_state(*this);
+    }
+    if (nb) {
+        neural_enqueue(*this);   // normals where the density pass found the wall penalty active
+        mark(kStageWallMlp);
     }
     launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes);
     mark(kStageForce);
