@@ -112,6 +112,34 @@ def cpu_baseline(scene, seconds: float, threads: int):
     }
 
 
+def kernel_name(stage: str, tuning: dict) -> str:
+    """The kernel a stage launched under the given variants."""
+    if stage == "force":
+        if tuning["hit_masks"]:
+            return f"k_force_masks<{tuning['hit_list']}>"
+        return "k_force" if tuning["force_lanes"] == 1 else f"k_force_coop<{tuning['force_lanes']}>"
+    if stage == "density":
+        if tuning["hit_masks"]:
+            return "k_density_masks"
+        return "k_density" if tuning["density_lanes"] == 1 else f"k_density_coop<{tuning['density_lanes']}>"
+    return f"k_{stage}"
+
+
+def profiled_traffic(config: str, kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
+    (calibrated FETCH_SIZE / WRITE_SIZE), or None."""
+    path = os.path.join(ROOT, "profiles", f"plain_{config}_step_kernels.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            for item in line.split(":", 1)[1].split():
+                k, v = item.split("=")
+                if k == kernel:
+                    return int(v)
+    return None
+
+
 def time_handles(sims, steps: int) -> float:
     """Wall time per step of separate handles stepped round-robin (each on
     its own stream), synchronised at both ends."""
@@ -132,6 +160,7 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
     # 1M scene with the learned boundary
     t0 = time.time()
     net = N.train_for_scene(sc.desc, steps=a.train_steps)
+    train_ms = net.time_train(100, 65536) / 100        # 100 more steps at the final learning rate
     sim = S.ParticleSimulation(sc).Init()
     rep = sim.attach_neural(net)
     sim.Advance(max(a.warmup, 1))
@@ -144,6 +173,12 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
         "vs_plain": round(plain_ms / ms, 4), "train_steps": a.train_steps,
         "cells": {"none": rep["cells_none"], "network": rep["cells_neural"], "grid": rep["cells_grid"]},
         "band_mean_error_h": round(rep["mean_error"] / float(sc.desc.h), 5), "kernel_variants": tun,
+        "train_ms_per_step": round(train_ms, 5),
+        "train_roofline": {"bound": "mfma", "kernel": "k_mlp_train + k_mlp_adam",
+                           "achieved": round(MLP_TRAIN_FLOP * 65536 / (train_ms / 1e3) / 1e12, 2),
+                           "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(MLP_TRAIN_FLOP * 65536 / (train_ms / 1e3) / 1e12 / MFMA_BF16_PEAK_TFLOPS, 5),
+                           "dtype": "bf16 (FP32 accumulate)"},
     }
     log(f"1M + learned boundary: {ms:.4f} ms/step ({time.time() - t0:.0f}s incl. training)")
     del sim
@@ -189,6 +224,28 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
     return forms
 
 
+# Algorithmic FLOPs of one training sample: forward 2 (99*64 + 64*64 + 64*64
+# + 64) = 29,184, training about three times the forward.
+MLP_FWD_FLOP = 2 * (99 * 64 + 64 * 64 + 64 * 64 + 64)
+MLP_TRAIN_FLOP = 3 * MLP_FWD_FLOP
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def train_profile(S, a) -> int:
+    from cudasphnew_amd import neural as N
+    sc = S.make_scene(a.config)
+    h = float(sc.desc.h)
+    net = N.NeuralSDF.for_scene(sc.desc, seed=1, out_scale=float(sc.desc.rest_distance))
+    net.set_samples(N.grid_samples(sc.desc, 1 << 22, near_frac=0.3, near_width=3 * h, seed=2))
+    net.train(5, 65536)
+    ms = net.time_train(a.train_profile, 65536) / a.train_profile
+    tflops = MLP_TRAIN_FLOP * 65536 / (ms / 1e3) / 1e12
+    print(json.dumps({"mlp_train_ms_per_step": round(ms, 5), "batch": 65536,
+                      "algorithmic_tflops": round(tflops, 2), "mfma_bf16_peak_tflops": MFMA_BF16_PEAK_TFLOPS,
+                      "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 5)}), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +259,8 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--no-forms", action="store_true", help="skip the learned-boundary and batched forms")
+    ap.add_argument("--train-profile", type=int, default=0,
+                    help="train the 1M scene's network for this many steps (batch 65536), report and exit")
     ap.add_argument("--train-steps", type=int, default=20000, help="training steps of the 1M scene's network")
     ap.add_argument("--batch-train-steps", type=int, default=2000, help="training steps per batch scene network")
     ap.add_argument("--batch-steps", type=int, default=50, help="timed steps of the batched forms")
@@ -216,6 +275,8 @@ def main(argv=None) -> int:
 
     import cudasphnew_amd as S
 
+    if a.train_profile:
+        return train_profile(S, a)
     if a.copy_probe:
         copy_probe(256 << 20)
     t0 = time.time()
@@ -255,14 +316,15 @@ def main(argv=None) -> int:
     algo_bytes = STAGE_BYTES_PER_PARTICLE[dom] * n + (8 * tc if dom in ("density", "force") else 0)
     t_dom = single[dom] / 1e3
     achieved = algo_bytes / t_dom / 1e9
+    kname = kernel_name(dom, tuning)
     roofline = {
         "bound": "hbm",
-        "kernel": f"k_{dom}",
+        "kernel": kname,
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": None,
+        "traffic": profiled_traffic(a.config, kname),
         "algorithmic_bytes_per_launch": int(algo_bytes),
         "avg_launch_ms": round(single[dom], 5),
         "touched_cells": tc,

@@ -439,21 +439,36 @@ __global__ __launch_bounds__(kThreads) void k_mlp_train(const float4* __restrict
 
 // Sums the partial gradients; with adam != 0 applies one Adam step
 // (Kingma & Ba 2015, Algorithm 1) to the FP32 master parameters and refreshes
-// the BF16 copies.  grad_out (optional) receives the summed gradient.
+// the BF16 copies.  grad_out (optional) receives the summed gradient.  A
+// block owns 64 parameters; its four wavefronts sum interleaved quarters of
+// the partials (four independent accumulators each) and meet in LDS.
+constexpr int kAdamParams = 64;
 __global__ __launch_bounds__(256) void k_mlp_adam(const float* __restrict__ partial, int nparts,
                                                   const float* __restrict__ loss_partial, float* theta, float* m,
                                                   float* v, Bf16Weights wb, float lr, float b1, float b2, float eps,
                                                   float c1, float c2, int adam, float* grad_out,
                                                   float* loss_out, float inv_batch) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0 && loss_out) {
+    __shared__ float sg[4][kAdamParams];
+    const int pl = threadIdx.x & (kAdamParams - 1), grp = threadIdx.x >> 6;
+    const int p = blockIdx.x * kAdamParams + pl;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && loss_out) {
         float l = 0.0f;
         for (int b = 0; b < nparts; ++b) l += loss_partial[b];
         *loss_out = l * inv_batch;
     }
-    if (p >= kParams) return;
-    float g = 0.0f;
-    for (int b = 0; b < nparts; ++b) g += partial[(size_t)b * kParams + p];
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (p < kParams) {
+        int b = grp;
+        for (; b + 12 < nparts; b += 16) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += partial[(size_t)(b + 4 * u) * kParams + p];
+        }
+        for (; b < nparts; b += 4) acc[0] += partial[(size_t)b * kParams + p];
+    }
+    sg[grp][pl] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (grp != 0 || p >= kParams) return;
+    const float g = (sg[0][pl] + sg[1][pl]) + (sg[2][pl] + sg[3][pl]);
     if (grad_out) grad_out[p] = g;
     if (!adam) return;
     float mm = b1 * m[p] + (1.0f - b1) * g;
@@ -1073,7 +1088,8 @@ static sph_status enqueue_pass(sph_mlp* h, uint32_t first, int32_t batch, bool a
             lr = (float)(h->cfg.lr_min + 0.5 * (h->cfg.lr - h->cfg.lr_min) * (1.0 + std::cos(3.141592653589793 * f)));
         }
     }
-    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + 255) / 256), dim3(256), 0, h->stream, h->partial, blocks,
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kParams + kAdamParams - 1) / kAdamParams), dim3(256), 0, h->stream,
+                       h->partial, blocks,
                        h->loss_partial, h->theta, h->m, h->v, h->wb, lr, h->cfg.beta1, h->cfg.beta2,
                        h->cfg.eps, c1, c2, adam ? 1 : 0, grad, loss_dev, 1.0f / (float)batch);
     if (hipGetLastError() != hipSuccess) return h->fail(SPH_DEVICE_ERROR, "training launch failed");
