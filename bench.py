@@ -182,6 +182,32 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
     }
     log(f"1M + learned boundary: {ms:.4f} ms/step ({time.time() - t0:.0f}s incl. training)")
     del sim
+    # mesh -> signed distance field, the reference's example configuration
+    # (main.cpp:91-103): icosphere fitted to [-0.75, 0.75]^3, compact support
+    # 0.017, dx = support / 3, rest distance = 2 support; 266^3 nodes
+    import numpy as np
+    from cudasphnew_amd import meshsdf as MS
+    meshes = np.load(os.path.join(ROOT, "tests", "golden", "meshes.npz"))
+    for name in ("icosphere", "bunny"):
+        v = MS.fit(meshes[f"{name}_vertices"], (-0.75,) * 3, (0.75,) * 3)
+        fcs = meshes[f"{name}_faces"]
+        cs = np.float32(0.017)
+        cfgm = MS.config(float(cs * np.float32(2)), float(cs), float(cs / np.float32(3)))
+        MS.generate(v, fcs, cfgm)                      # warm-up
+        ms_runs = [MS.generate(v, fcs, cfgm)["ms"] for _ in range(3)]
+        bm = MS.generate(v, fcs, cfgm)
+        nodes = int(np.prod(bm["dims"]))
+        entry = {"mesh": name, "triangles": int(fcs.shape[0]), "dims": list(bm["dims"]), "nodes": nodes,
+                 "nodes_within_maxdist": int(bm["present"].sum()), "ms": round(min(ms_runs), 4),
+                 "nodes_per_s": round(nodes / (min(ms_runs) / 1e3), 1)}
+        if name == "icosphere" and not a.no_cpu_baseline:
+            from oracle import oracle as O
+            t1 = time.perf_counter()
+            O.mesh_sdf(v, fcs, cfgm.rest_distance, cfgm.compact_support, cfgm.dx, bm["dims"], bm["lo"], bm["hi"])
+            entry["cpu_oracle_ms"] = round((time.perf_counter() - t1) * 1e3, 1)
+            entry["cpu_oracle"] = "FP64 restatement (oracle/mesh_oracle.cpp), OpenMP"
+        forms[f"boundary_map_{name}"] = entry
+        log(f"boundary map {name}: {entry['dims']} in {entry['ms']} ms")
     # 64 scenes
     t0 = time.time()
     scenes = [S.make_scene("batch", index=i, seed=1) for i in range(64)]

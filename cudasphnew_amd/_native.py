@@ -170,6 +170,15 @@ class SphNeuralReport(C.Structure):
     ]
 
 
+class SphMesh(C.Structure):
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("num_vertices", C.c_uint32),
+                ("faces", C.POINTER(C.c_uint32)), ("num_faces", C.c_uint32)]
+
+
+class SphBoundaryMapConfig(C.Structure):
+    _fields_ = [("rest_distance", C.c_float), ("compact_support", C.c_float), ("dx", C.c_float)]
+
+
 class SphError(RuntimeError):
     """Raised for a non-OK sph_status (the reference throws
     std::runtime_error from CUDA_SAFE_CALL, util.h:24-39)."""
@@ -252,6 +261,15 @@ SIGNATURES = [
     ("sph_detach_neural_boundary", _st, [_vp]),
     ("sph_batch_attach_neural_boundary", _st, [_vp, C.c_int32, _vp, fptr, C.c_size_t,
                                                C.POINTER(SphNeuralConfig), C.POINTER(SphNeuralReport)]),
+    ("sph_obj_read", _st, [C.c_char_p, fptr, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("sph_mesh_fit", _st, [fptr, C.c_uint32, f3, f3]),
+    ("sph_boundary_map_domain", _st, [C.POINTER(SphMesh), C.c_float, i3, f3, f3]),
+    ("sph_boundary_map_generate", _st, [C.POINTER(SphMesh), C.POINTER(SphBoundaryMapConfig), fptr, fptr,
+                                        C.POINTER(C.c_uint8), fptr]),
+    ("sph_boundary_map_save", _st, [C.c_char_p, C.POINTER(SphBoundaryMapConfig), i3, f3, f3, fptr, fptr,
+                                    C.POINTER(C.c_uint8)]),
+    ("sph_boundary_map_load", _st, [C.c_char_p, C.POINTER(SphBoundaryMapConfig), i3, f3, f3, fptr, fptr,
+                                    C.POINTER(C.c_uint8)]),
     ("sph_abi_version", C.c_int32, []),
     ("sph_device_count", C.c_int32, []),
 ]

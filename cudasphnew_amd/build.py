@@ -50,6 +50,7 @@ SOURCES = [
     ("twoscale.hip", "device"),
     ("sim.hip", "device"),
     ("neural.hip", "device"),
+    ("mesh.hip", "device"),
 ]
 
 

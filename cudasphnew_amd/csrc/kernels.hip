@@ -117,11 +117,18 @@ __device__ __forceinline__ Rows neighbour_rows(const DevScene& S, float4 xi) {
 // the clamped range (e.g. at the grid edge, or the reference's wrapped upper
 // coordinate).
 __device__ __forceinline__ void row_slots(const int2* __restrict__ range, const DevScene& S, const Rows& R, int dk,
-                                          int dj, int& b, int& e) {
+                                          int dj, float4 xi, int& b, int& e) {
     const int k = R.lz + dk, j = R.ly + dj;
     b = 0;
     e = 0;
     if (k > R.hz || j > R.hy) return;
+    // a row whose y-z cell rectangle is farther than h from the particle
+    // holds no neighbour (its candidates would add exact zeros); the 1e-5
+    // margin covers the rounding of the cell bounds and of the keys
+    const float y0 = S.oy + (float)j * S.cell, z0 = S.oz + (float)k * S.cell;
+    const float dy = fmaxf(fmaxf(y0 - xi.y, xi.y - (y0 + S.cell)), 0.0f);
+    const float dz = fmaxf(fmaxf(z0 - xi.z, xi.z - (z0 + S.cell)), 0.0f);
+    if ((dy * dy + dz * dz) * 0.99999f > S.h2) return;
     uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
     row_run(range, rowbase, R.lx, R.hx, b, e);
     if (e < b) { b = 0; e = 0; }
@@ -285,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ p
     for (int dk = 0; dk < 3; ++dk)
         for (int dj = 0; dj < 3; ++dj) {
             int b, e;
-            row_slots(range, S, R, dk, dj, b, e);
+            row_slots(range, S, R, dk, dj, xi, b, e);
 #pragma unroll 4
             for (int q = b; q < e; ++q) {
                 float4 xj = pos[q];
@@ -328,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restric
             for (int dk = 0; dk < 3; ++dk)
                 for (int dj = 0; dj < 3; ++dj) {
                     int b, e;
-                    row_slots(range, S, R, dk, dj, b, e);
+                    row_slots(range, S, R, dk, dj, xi, b, e);
                     for (int q = b + c; q < e; q += L) {
                         float4 xj = pos[q];
                         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -377,7 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
     for (int dk = 0; dk < 3; ++dk)
         for (int dj = 0; dj < 3; ++dj) {
             int b, e;
-            row_slots(range, S, R, dk, dj, b, e);
+            row_slots(range, S, R, dk, dj, xi, b, e);
             for (int base = b; base < e; base += 64) {
                 const int cnt = min(64, e - base);
                 uint64_t hits = 0;
@@ -444,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
         for (int dk = 0; dk < 3; ++dk)
             for (int dj = 0; dj < 3; ++dj) {
                 int b = 0, e = 0;
-                if (valid) row_slots(range, S, R, dk, dj, b, e);
+                if (valid) row_slots(range, S, R, dk, dj, xi, b, e);
                 for (int q = b + c; __any(q < e); q += L) {
                     bool hit = false;
                     if (q < e) {
@@ -476,7 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
             for (int dk = 0; dk < 3; ++dk)
                 for (int dj = 0; dj < 3; ++dj) {
                     int b, e;
-                    row_slots(range, S, R, dk, dj, b, e);
+                    row_slots(range, S, R, dk, dj, xi, b, e);
                     for (int q = b; q < e; ++q) {
                         float4 xj = pos[q];
                         float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -537,19 +544,34 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
         int b, e;
-        row_slots(range, S, R, row / 3, row % 3, b, e);
+        row_slots(range, S, R, row / 3, row % 3, xi, b, e);
         const int cnt = e - b;
-        const int c64 = min(cnt, 64);
-        uint64_t m = 0;
+        // force neighbour test 0 < r2 <= h2 as one unsigned compare of the
+        // bit patterns (r2 >= +0 is a sum of squares; 0 - 1 wraps above
+        // every finite pattern, NaN lies above h2's); mask bits in two
+        // 32-bit halves so that the shift is a 32-bit one
+        const uint32_t h2b = __float_as_uint(h2);
+        uint32_t mlo = 0, mhi = 0;
+        const int c32 = min(cnt, 32), c64 = min(cnt, 64);
 #pragma unroll 4
-        for (int t = 0; t < c64; ++t) {
+        for (int t = 0; t < c32; ++t) {
             float4 xj = pos[b + t];
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float d = fmaxf(h2 - r2, 0.0f);
             sum += d * d * d;
-            m |= (uint64_t)(r2 <= h2 && r2 > 0.0f) << t;
+            mlo |= (uint32_t)(__float_as_uint(r2) - 1u < h2b) << t;
         }
+#pragma unroll 4
+        for (int t = 32; t < c64; ++t) {
+            float4 xj = pos[b + t];
+            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float d = fmaxf(h2 - r2, 0.0f);
+            sum += d * d * d;
+            mhi |= (uint32_t)(__float_as_uint(r2) - 1u < h2b) << (t - 32);
+        }
+        const uint64_t m = ((uint64_t)mhi << 32) | mlo;
         for (int t = 64; t < cnt; ++t) {
             float4 xj = pos[b + t];
             float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -597,7 +619,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
         int b, e;
-        row_slots(range, S, R, row / 3, row % 3, b, e);
+        row_slots(range, S, R, row / 3, row % 3, xi, b, e);
         s_base[row][tid] = b;
         if (e - b <= 64) {
             uint64_t m = masks[(size_t)row * n + s];

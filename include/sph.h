@@ -519,6 +519,57 @@ sph_status sph_batch_attach_neural_boundary(sph_batch* b, int32_t scene,
                                             const sph_neural_config* cfg,
                                             sph_neural_report* report);
 
+/* ---- mesh -> signed distance field ------------------------------------------
+ * BoundaryMap (boundary_map.h:37-122, boundary_map.cpp): the signed
+ * distance of a triangle mesh on a grid over the mesh's bounding box, with
+ * the unit normal towards the node, stored where the distance is within
+ * maxDist = max(restDistance, compactSupport) of the surface; negative
+ * inside.  Generated on the GPU. */
+typedef struct sph_mesh {
+    const float* vertices;      /* xyz per vertex */
+    uint32_t num_vertices;
+    const uint32_t* faces;      /* three 0-based vertex indices per triangle */
+    uint32_t num_faces;
+} sph_mesh;
+/* BoundaryMapConfiguration (boundary_map.h:25-31). */
+typedef struct sph_boundary_map_config {
+    float rest_distance, compact_support, dx;
+} sph_boundary_map_config;
+/* OBJ reader (TriangleMesh(filename), triangle_mesh.cpp): "v x y z" and
+ * "f a b c" lines (1-based; polygons fanned).  Two calls: with NULL arrays
+ * (or too small capacities in *num_vertices / *num_faces) it returns the
+ * counts. */
+sph_status sph_obj_read(const char* path, float* vertices,
+                        uint32_t* num_vertices, uint32_t* faces,
+                        uint32_t* num_faces);
+/* TriangleMesh::fit: scale per axis about the bounding-box centre to the
+ * target box, then move the box corner to lo. */
+sph_status sph_mesh_fit(float* vertices, uint32_t num_vertices,
+                        const float lo[3], const float hi[3]);
+/* AddCanvas domain (boundary_map.cpp:60-86): dims = ceil((v2 - v1)/dx) + 1,
+ * lo = v1, hi = v1 + (dims - 1) dx. */
+sph_status sph_boundary_map_domain(const sph_mesh* mesh, float dx,
+                                   int32_t dims[3], float lo[3], float hi[3]);
+/* AddCanvas + computeSignedDistanceField (boundary_map.cpp:369-490) on the
+ * GPU: dense outputs over the domain (distance; normal xyz; present = 1
+ * where a distance <= maxDist was found; absent nodes hold
+ * (restDistance, 0, 0, 0) as after BoundaryMap::Load).  ms (optional):
+ * device time of the two kernels. */
+sph_status sph_boundary_map_generate(const sph_mesh* mesh,
+                                     const sph_boundary_map_config* cfg,
+                                     float* distance, float* normal,
+                                     uint8_t* present, float* ms);
+/* BoundaryMap::Save / Load text format (boundary_map.cpp:161-279). */
+sph_status sph_boundary_map_save(const char* path,
+                                 const sph_boundary_map_config* cfg,
+                                 const int32_t dims[3], const float lo[3],
+                                 const float hi[3], const float* distance,
+                                 const float* normal, const uint8_t* present);
+sph_status sph_boundary_map_load(const char* path, sph_boundary_map_config* cfg,
+                                 int32_t dims[3], float lo[3], float hi[3],
+                                 float* distance, float* normal,
+                                 uint8_t* present);
+
 /* ---- library info ----------------------------------------------------- */
 int32_t sph_abi_version(void);
 /* Number of visible HIP devices (0 on a machine without a GPU). */

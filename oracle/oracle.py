@@ -223,3 +223,53 @@ def struct_layout():
     out = (C.c_size_t * 10)()
     lib.orc_struct_layout(out)
     return list(out)
+
+
+def _mesh_sig():
+    u8p = C.POINTER(C.c_uint8)
+    u32p = C.POINTER(C.c_uint32)
+    lib.orc_mesh_sdf.restype = None
+    lib.orc_mesh_sdf.argtypes = [C.POINTER(C.c_float), C.c_uint32, u32p, C.c_uint32, C.c_float, C.c_float,
+                                 C.c_float, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), u8p]
+    lib.orc_point_triangle.restype = None
+    lib.orc_point_triangle.argtypes = [C.POINTER(C.c_double), C.c_size_t, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+    lib.orc_inside.restype = None
+    lib.orc_inside.argtypes = [C.POINTER(C.c_float), u32p, C.c_uint32, C.POINTER(C.c_float), C.c_size_t, u8p]
+
+
+_mesh_sig()
+
+
+def mesh_sdf(vertices, faces, rest: float, support: float, dx: float, dims, lo, hi):
+    """FP64 signed distance field of a mesh on the given domain: distance,
+    normal, present arrays shaped (nz, ny, nx)."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(faces, np.uint32)
+    shape = (dims[2], dims[1], dims[0])
+    d = np.empty(shape, np.float64)
+    n = np.empty(shape + (3,), np.float64)
+    p = np.empty(shape, np.uint8)
+    lib.orc_mesh_sdf(_fp(v), v.shape[0], f.ctypes.data_as(C.POINTER(C.c_uint32)), f.shape[0], float(dx),
+                     float(rest), float(support), (C.c_int32 * 3)(*dims), (C.c_float * 3)(*lo),
+                     (C.c_float * 3)(*hi), _dp(d), _dp(n), p.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return d, n, p
+
+
+def point_triangle(points, tri):
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tri, np.float64).reshape(9)
+    out = np.empty(p.shape[0])
+    lib.orc_point_triangle(_dp(p), p.shape[0], _dp(t), _dp(out))
+    return out
+
+
+def inside(vertices, faces, points):
+    v = np.ascontiguousarray(vertices, np.float32)
+    f = np.ascontiguousarray(faces, np.uint32)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    out = np.empty(p.shape[0], np.uint8)
+    lib.orc_inside(_fp(v), f.ctypes.data_as(C.POINTER(C.c_uint32)), f.shape[0], _fp(p), p.shape[0],
+                   out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return out.astype(bool)
