@@ -141,13 +141,21 @@ struct PairSums {
     float fpx, fpy, fpz, fvx, fvy, fvz, cgx, cgy, cgz, cl;
 };
 
+// GUARD: the caller has not tested r2 > 0; a pair at distance 0 then adds
+// exact zeros (inverse distance and 1/rho_j taken as 0).
+template <bool GUARD = false>
 __device__ __forceinline__ void pair_terms(PairSums& a, float4 xi, float4 vi, float rhoi, float Ai, float h, float h2,
                                            float stiff, float kr0, float4 xj, float4 vj) {
     float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
     float r2 = dx * dx + dy * dy + dz * dz;
     float inv_r = __builtin_amdgcn_rsqf(r2);
-    float rn = r2 * inv_r;
     float invj = __builtin_amdgcn_rcpf(vj.w);
+    if (GUARD) {
+        const bool ok = r2 > 0.0f;
+        inv_r = ok ? inv_r : 0.0f;
+        invj = ok ? invj : 0.0f;
+    }
+    float rn = r2 * inv_r;
     float pj = stiff * vj.w - kr0;
     float hr = h - rn;
     float qd = h2 - r2;
@@ -521,75 +529,151 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
 
 // Hit-mask pipeline.  The density pass already evaluates the distance of
 // every candidate of the nine row runs; it records, per particle and row,
-// which of the first 64 candidates are force neighbours (0 < r^2 <= h^2) in a
-// 64-bit mask (masks[row * n + slot], coalesced).  The force pass then skips
-// the distance tests: it expands the masks into a per-thread hit list in LDS
-// and runs the pair terms over the list, so a wavefront's trip count is the
+// which of the first 64 candidates are force neighbours in a 64-bit mask
+// (masks[row * n + slot], coalesced).  The force pass then skips the
+// distance tests: it expands the masks into a per-thread hit list in LDS and
+// runs the pair terms over the list, so a wavefront's trip count is the
 // largest neighbour count of its lanes rather than the sum over rows of the
-// largest per-row count.  Rows longer than 64 candidates (strong compression)
-// are handled by the force pass directly.
+// largest per-row count.  Rows longer than 64 candidates (strong
+// compression) are handled by the force pass directly.
+//
+// Mask bits: r^2 <= h^2, without the particle's own slot.  A distinct
+// particle at distance 0 keeps its bit; the force pass gives it no
+// contribution (the pair terms require 0 < r^2, as the reference's
+// `rn > 0`).
+
+// Neighbour bits of c <= 32 candidates from the sign bits of h2 - r2 shifted
+// in newest-first (bit 0 = candidate c - 1, set = farther than h): reverse,
+// invert, drop the unused high part.
+__device__ __forceinline__ uint32_t near_bits(uint32_t far, int c) {
+    return c > 0 ? __builtin_bitreverse32(~far) >> (32 - c) : 0u;
+}
+
+// One candidate of the density pass: poly6 sum and the "farther than h" bit.
+// d = h2 - r2 is negative exactly when r2 > h2 (IEEE subtraction keeps the
+// sign of the exact difference, and h2 - h2 = +0), so its sign bit is that
+// bit and one funnel shift per candidate builds the mask.
+__device__ __forceinline__ void density_candidate(float xjx, float xjy, float xjz, float4 xi, float h2, float& sum,
+                                                  uint32_t& far) {
+    float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float d = h2 - r2;
+    far = __builtin_amdgcn_alignbit(far, __float_as_uint(d), 31);
+    d = fmaxf(d, 0.0f);
+    sum += d * d * d;
+}
+
+// The candidates [0, cnt) of one row run; returns the force-neighbour mask
+// of the first 64 (candidate t at bit t).  at(t) -> (x, y, z) of candidate t.
+template <class At>
+__device__ __forceinline__ uint64_t density_row(const At& at, int cnt, float4 xi, float h2, float& sum) {
+    uint32_t far_lo = 0, far_hi = 0;
+    const int c32 = min(cnt, 32), c64 = min(cnt, 64);
+#pragma unroll 4
+    for (int t = 0; t < c32; ++t) {
+        const float3 p = at(t);
+        density_candidate(p.x, p.y, p.z, xi, h2, sum, far_lo);
+    }
+#pragma unroll 4
+    for (int t = 32; t < c64; ++t) {
+        const float3 p = at(t);
+        density_candidate(p.x, p.y, p.z, xi, h2, sum, far_hi);
+    }
+    for (int t = 64; t < cnt; ++t) {
+        const float3 p = at(t);
+        float dx = p.x - xi.x, dy = p.y - xi.y, dz = p.z - xi.z;
+        float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
+        sum += d * d * d;
+    }
+    return ((uint64_t)near_bits(far_hi, c64 - 32) << 32) | near_bits(far_lo, c32);
+}
+
+// Window of candidate positions per wavefront, in LDS (x, y, z planes).  The
+// 64 particles of a wavefront are consecutive slots (a few neighbouring cells
+// of one cell row, as a rule), so for each of the nine row offsets their
+// runs lie in one short slot window [lo, hi).  The wavefront copies each
+// window once with coalesced loads, and the candidate loop reads LDS: one
+// global load per particle and row in place of one per particle and
+// candidate, which leaves the pass bound by its arithmetic instead of the
+// texture data path.  A row whose window does not fit (the wavefront
+// straddles two cell rows or two scenes) reads global memory as before.
+constexpr int kWin = 768;
+
 __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                           const int2* __restrict__ range, uint32_t n,
                                                           const DevScene* __restrict__ scenes,
                                                           const uint32_t* __restrict__ begins, int nscenes,
                                                           const float4* __restrict__ grid4,
                                                           uint64_t* __restrict__ masks, NeuralRefs nr) {
-    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+    __shared__ float s_win[kBlock / 64][3 * kWin];
+    const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if ((s0 & ~63u) >= n) return;   // the whole wavefront lies past the end
+    // every lane of a live wavefront takes part in the window copies; lanes
+    // past the end repeat the last particle and store nothing
+    const bool live = s0 < n;
+    const uint32_t s = live ? s0 : n - 1;
+    const int lane = threadIdx.x & 63;
+    float* win = s_win[threadIdx.x >> 6];
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
-    float4 xi = pos[s];
+    const float4 xi = pos[s];
     const float h2 = S.h2;
     Rows R = neighbour_rows(S, xi);
+    int rb[9], re[9], wofs[9];
+    bool inlds[9];
+    int fill = 0;
+#pragma unroll
+    for (int row = 0; row < 9; ++row) {
+        row_slots(range, S, R, row / 3, row % 3, xi, rb[row], re[row]);
+        const bool any = re[row] > rb[row];
+        const int lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(any ? rb[row] : 0x7fffffff));
+        const int hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(any ? re[row] : -1));
+        inlds[row] = hi > lo && hi - lo <= kWin - fill;
+        wofs[row] = fill - lo;   // LDS index of slot q: q + wofs
+        if (inlds[row]) {
+            for (int i = lane; i < hi - lo; i += 64) {
+                const float4 p = pos[lo + i];
+                win[fill + i] = p.x;
+                win[kWin + fill + i] = p.y;
+                win[2 * kWin + fill + i] = p.z;
+            }
+            fill += hi - lo;
+        }
+    }
+    // the window is private to the wavefront: its own LDS writes, in
+    // program order, are visible to its later reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     float sum = 0.0f;
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
-        int b, e;
-        row_slots(range, S, R, row / 3, row % 3, xi, b, e);
-        const int cnt = e - b;
-        // force neighbour test 0 < r2 <= h2 as one unsigned compare of the
-        // bit patterns (r2 >= +0 is a sum of squares; 0 - 1 wraps above
-        // every finite pattern, NaN lies above h2's); mask bits in two
-        // 32-bit halves so that the shift is a 32-bit one
-        const uint32_t h2b = __float_as_uint(h2);
-        uint32_t mlo = 0, mhi = 0;
-        const int c32 = min(cnt, 32), c64 = min(cnt, 64);
-#pragma unroll 4
-        for (int t = 0; t < c32; ++t) {
-            float4 xj = pos[b + t];
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float d = fmaxf(h2 - r2, 0.0f);
-            sum += d * d * d;
-            mlo |= (uint32_t)(__float_as_uint(r2) - 1u < h2b) << t;
+        const int b = rb[row], cnt = re[row] - rb[row];
+        uint64_t m;
+        if (inlds[row]) {
+            const float* w = win + (b + wofs[row]);
+            m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
+                            sum);
+        } else {
+            m = density_row([&](int t) { const float4 p = pos[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
+                            xi, h2, sum);
         }
-#pragma unroll 4
-        for (int t = 32; t < c64; ++t) {
-            float4 xj = pos[b + t];
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float d = fmaxf(h2 - r2, 0.0f);
-            sum += d * d * d;
-            mhi |= (uint32_t)(__float_as_uint(r2) - 1u < h2b) << (t - 32);
-        }
-        const uint64_t m = ((uint64_t)mhi << 32) | mlo;
-        for (int t = 64; t < cnt; ++t) {
-            float4 xj = pos[b + t];
-            float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float d = fmaxf(h2 - r2, 0.0f);
-            sum += d * d * d;
-        }
-        masks[(size_t)row * n + s] = m;
+        // the particle's own slot is not a force neighbour
+        const uint32_t self = s - (uint32_t)b;
+        if (self < (uint32_t)min(cnt, 64)) m &= ~(1ull << self);
+        if (live) masks[(size_t)row * n + s] = m;
     }
-    float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
-    reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+    if (live) {
+        float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
+        reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+    }
 }
 
 // The hit list holds 16-bit entries (row << 6 | position in the row's run)
 // and the nine run starts per thread, so that four blocks fit the LDS of a
-// CU; the pair loop fetches the next neighbour's position and velocity
-// before it evaluates the current one, so the gathers overlap the
-// arithmetic.
+// CU.  Entries past the capacity go to a spare list row and are evaluated
+// afterwards from the masks, so that the expansion loop carries no pair
+// arithmetic; the pair loop keeps the gathers of the next three neighbours
+// in flight while it evaluates the current one.
 template <int CAP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
@@ -601,11 +685,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     NeuralRefs nr) {
     // one column per thread: no barrier needed, consecutive threads hit
     // consecutive banks
-    __shared__ uint16_t s_list[CAP][kBlock];
+    __shared__ uint16_t s_list[CAP + 1][kBlock];
     __shared__ int s_base[9][kBlock];
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    uint64_t mk[9];
+#pragma unroll
+    for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
     const float4 xi = pos[s];
     const float4 vi = vel[s];
     const float rhoi = vi.w;
@@ -615,44 +702,83 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
     Rows R = neighbour_rows(S, xi);
     PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int cnt = 0;
+    uint32_t long_rows = 0;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
         int b, e;
         row_slots(range, S, R, row / 3, row % 3, xi, b, e);
         s_base[row][tid] = b;
-        if (e - b <= 64) {
-            uint64_t m = masks[(size_t)row * n + s];
-            while (m) {
-                const int t = __ffsll((unsigned long long)m) - 1;
-                m &= m - 1;
-                if (cnt < CAP) s_list[cnt][tid] = (uint16_t)(row << 6 | t);
-                else pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
-                ++cnt;
-            }
-        } else {
-            for (int q = b; q < e; ++q) {
-                float4 xj = pos[q];
-                float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                if (r2 <= h2 && r2 > 0.0f) pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
-            }
+        if (e - b > 64) {
+            long_rows |= 1u << row;
+            continue;
+        }
+        uint32_t lo = (uint32_t)mk[row], hi = (uint32_t)(mk[row] >> 32);
+        while (lo) {
+            const int t = __builtin_ctz(lo);
+            lo &= lo - 1;
+            s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
+            ++cnt;
+        }
+        while (hi) {
+            const int t = __builtin_ctz(hi) + 32;
+            hi &= hi - 1;
+            s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
+            ++cnt;
         }
     }
     const int m = min(cnt, CAP);
     if (m > 0) {
-        int ent = s_list[0][tid];
-        int q = s_base[ent >> 6][tid] + (ent & 63);
-        float4 xj = pos[q], vj = vel[q];
-        for (int k = 1; k < m; ++k) {
-            ent = s_list[k][tid];
-            q = s_base[ent >> 6][tid] + (ent & 63);
+        // the gathers of entries k+1..k+3 are issued before the pair terms
+        // of entry k (short lists repeat their last entry, which is loaded
+        // but not evaluated)
+        auto slot = [&](int k) {
+            const int ent = s_list[min(k, m - 1)][tid];
+            return s_base[ent >> 6][tid] + (ent & 63);
+        };
+        int q = slot(0);
+        float4 x0 = pos[q], v0 = vel[q];
+        q = slot(1);
+        float4 x1 = pos[q], v1 = vel[q];
+        q = slot(2);
+        float4 x2 = pos[q], v2 = vel[q];
+        for (int k = 3; k < m; ++k) {
+            q = slot(k);
             const float4 xn = pos[q], vn = vel[q];
-            pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vj);
-            xj = xn;
-            vj = vn;
+            pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+            x0 = x1; v0 = v1;
+            x1 = x2; v1 = v2;
+            x2 = xn; v2 = vn;
         }
-        pair_terms(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vj);
+        pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+        if (m > 1) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
+        if (m > 2) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
+    }
+    if (cnt > CAP || long_rows) {
+        // rare: hit lists longer than the capacity, rows longer than 64
+        int k = 0;
+#pragma unroll 1
+        for (int row = 0; row < 9; ++row) {
+            int b, e;
+            if (long_rows >> row & 1u) {
+                row_slots(range, S, R, row / 3, row % 3, xi, b, e);
+                for (int q = b; q < e; ++q) {
+                    float4 xj = pos[q];
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 <= h2 && r2 > 0.0f) pair_terms<false>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
+                }
+                continue;
+            }
+            b = s_base[row][tid];
+            uint64_t mm = masks[(size_t)row * n + s];
+            while (mm) {
+                const int t = __ffsll((unsigned long long)mm) - 1;
+                mm &= mm - 1;
+                if (k >= CAP) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
+                ++k;
+            }
+        }
     }
     force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
                  status, nr);
