@@ -140,6 +140,27 @@ def profiled_traffic(config: str, kernel: str):
     return None
 
 
+def profiled_busy(config: str, kernel: str):
+    """VALU / texture-address / texture-data busy fractions of a kernel from
+    the committed rocprofv3 summary table, or None."""
+    path = os.path.join(ROOT, "profiles", f"plain_{config}_step_kernels.md")
+    if not os.path.exists(path):
+        return None
+    head = None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if line.startswith("| kernel |"):
+            head = cells
+        elif head and line.startswith("|") and cells and cells[0] == kernel:
+            row = dict(zip(head, cells))
+            try:
+                return {"valu": float(row["VALU busy"]), "ta": float(row["TA busy"]),
+                        "td": float(row["TD busy"])}
+            except (KeyError, ValueError):
+                return None
+    return None
+
+
 def time_handles(sims, steps: int) -> float:
     """Wall time per step of separate handles stepped round-robin (each on
     its own stream), synchronised at both ends."""
@@ -351,6 +372,10 @@ def main(argv=None) -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": profiled_traffic(a.config, kname),
+        # the step's neighbour kernels are bound by arithmetic and by the
+        # texture path of their gathers, not by HBM bytes: busy fractions
+        # of the VALU and texture units from the same profile
+        "profiled_busy": profiled_busy(a.config, kname),
         "algorithmic_bytes_per_launch": int(algo_bytes),
         "avg_launch_ms": round(single[dom], 5),
         "touched_cells": tc,
