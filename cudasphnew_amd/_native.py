@@ -113,6 +113,15 @@ class SphScene(C.Structure):
     ]
 
 
+class SphRenderView(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p), ("sub_positions", C.c_void_p),
+        ("default_ids", C.c_void_p), ("sub_ids", C.c_void_p),
+        ("num_particles", C.c_uint32), ("num_default", C.c_uint32),
+        ("num_sub_regular", C.c_uint32), ("num_sub_boundary", C.c_uint32),
+    ]
+
+
 class SphCounts(C.Structure):
     _fields_ = [
         ("num_particles", C.c_uint32), ("num_split", C.c_uint32),
@@ -211,6 +220,8 @@ SIGNATURES = [
     ("sph_read_positions", _st, [_vp, fptr, C.c_size_t]),
     ("sph_read_field", _st, [_vp, C.c_int, _vp, C.c_size_t]),
     ("sph_map_positions", _st, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    ("sph_map_render", _st, [_vp, C.POINTER(SphRenderView)]),
+    ("sph_unmap_render", _st, [_vp]),
     ("sph_get_counts", _st, [_vp, C.POINTER(SphCounts)]),
     ("sph_num_cells", C.c_size_t, [_vp]),
     ("sph_device_bytes", C.c_size_t, [_vp]),

@@ -97,6 +97,10 @@ struct Engine {
     sph_tuning tuning = {1, 1, 0, 48};
     uint64_t* masks = nullptr;   // [9][N] per-row force-neighbour masks
 
+    // render interop buffers, allocated on the first map
+    float4* render_pos = nullptr;   // [N] by original id
+    float4* render_sub = nullptr;   // [8N] by sub-particle id
+
     ~Engine();
     void drop_graph();
     sph_status set_tuning(const sph_tuning& t);
@@ -113,6 +117,7 @@ struct Engine {
     sph_status alloc(T** p, size_t count);
     sph_status read_field(int scene, sph_field f, void* dst, size_t bytes_out);
     sph_status upload_params(int scene, const sph_params& p);
+    sph_status map_render(sph_render_view* out);
 };
 
 DevScene make_dev_scene(const sph_params& P, const sph_boundary& B);
@@ -125,6 +130,9 @@ sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
 sph_status twoscale_counts(Engine& e, sph_counts* out);
 size_t twoscale_bytes(const Engine& e);
 uint8_t* twoscale_final_state(Engine& e);
+// Sub-particle positions of the last step by sub id into out (8N float4),
+// and the default / active sub id lists and counts into the view.
+sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view);
 
 // Learned boundary (neural.hip).
 sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, size_t nval,

@@ -3,7 +3,9 @@
 // Data layout: particles live in SoA float4 streams that are re-ordered by
 // cell every step, so that neighbour gathers are coalesced and every
 // 3-cell x-row of a neighbourhood is one contiguous run of slots.
-//   pos (x, y, z, 0), vel (vx, vy, vz, density), acc (ax, ay, az, 0).
+//   pos (x, y, z, density), vel (vx, vy, vz, density), acc (ax, ay, az, 0);
+//   the density is the one of the step that wrote the record (0 before the
+//   first step).
 // The reference keeps AoS arrays in original-id order and gathers through
 // the sorted id list instead (particle_simulation.cu:206,327).
 //
@@ -81,6 +83,17 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     oid_b[dst] = me;
     if (state_a) state_b[dst] = state_a[src];
     if (slot_of_oid) slot_of_oid[me] = dst;
+}
+
+// Positions by original id for a renderer: (x, y, z, density).
+__global__ __launch_bounds__(kBlock) void k_export_positions(uint32_t n, const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             const uint32_t* __restrict__ oid,
+                                                             float4* __restrict__ out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    out[oid[i]] = make_float4(p.x, p.y, p.z, vel[i].w);
 }
 
 // Contiguous slot run of the cells [i0, i1] of one x-row (cells of a row
@@ -262,7 +275,7 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
     if (!(isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(vx) && isfinite(vy) && isfinite(vz)))
         flags |= kStatusNonFinite;
     if (flags) atomicOr(status, flags);
-    pos_out[s] = make_float4(px, py, pz, 0.0f);
+    pos_out[s] = make_float4(px, py, pz, rhoi);
     vel_out[s] = make_float4(vx, vy, vz, rhoi);
     key_out[s] = kk;
     oid_out[s] = oid_b[s];
@@ -869,6 +882,12 @@ void launch_density(hipStream_t st, const StepBuffers& b, int lanes) {
 #define SPH_FORCE_ARGS_M                                                                                            \
     b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
         b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.masks, b.nr
+
+void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, const float4* vel, const uint32_t* oid,
+                             float4* out) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_export_positions, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, n, pos, vel, oid, out);
+}
 
 void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
     dim3 g(blocks_for(b.n, kBlock)), t(kBlock);

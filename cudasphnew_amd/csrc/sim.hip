@@ -108,7 +108,8 @@ Engine::~Engine() {
     drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
-                    perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode};
+                    perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode,
+                    render_pos, render_sub};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     sorter.release();
@@ -651,6 +652,34 @@ sph_status Engine::read_field(int s, sph_field f, void* dst, size_t nbytes) {
     }
 }
 
+// Render interop: the storage arrays hold the last step's state in cell
+// order; export the positions by original id (and the sub-particles by sub
+// id) into buffers a renderer can index with the id lists.
+sph_status Engine::map_render(sph_render_view* out) {
+    if (nscenes != 1) return fail(SPH_INVALID_ARG, "render interop is per handle (one scene)");
+    if (!render_pos) {
+        sph_status st = alloc(&render_pos, N);
+        if (st != SPH_OK) return st;
+        bytes += sizeof(float4) * (size_t)N;
+    }
+    std::memset(out, 0, sizeof(*out));
+    launch_export_positions(stream, N, pos_a, vel_a, oid_a, render_pos);
+    sph_status st = check_launch("render export");
+    if (st != SPH_OK) return st;
+    out->positions = reinterpret_cast<const float*>(render_pos);
+    out->num_particles = N;
+    if (ts && last_two_scale) {
+        if (!render_sub) {
+            st = alloc(&render_sub, (size_t)8 * N);
+            if (st != SPH_OK) return st;
+            bytes += sizeof(float4) * (size_t)8 * N;
+        }
+        if ((st = twoscale_export(*this, render_sub, out)) != SPH_OK) return st;
+    }
+    if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "render export failed");
+    return SPH_OK;
+}
+
 }  // namespace sph
 
 using sph::Engine;
@@ -747,6 +776,13 @@ sph_status sph_map_positions(sph_handle* h, const float** dev_pos4, const uint32
     *dev_oid = h->e.oid_a;
     return SPH_OK;
 }
+
+sph_status sph_map_render(sph_handle* h, sph_render_view* out) {
+    if (!h || !out) return SPH_INVALID_ARG;
+    return h->e.map_render(out);
+}
+
+sph_status sph_unmap_render(sph_handle* h) { return h ? SPH_OK : SPH_INVALID_ARG; }
 
 sph_status sph_get_counts(sph_handle* h, sph_counts* out) {
     if (!h || !out) return SPH_INVALID_ARG;

@@ -472,6 +472,18 @@ __global__ void k_ts_prev(TSCounts* __restrict__ cnt) {
     if (threadIdx.x == 0 && blockIdx.x == 0) cnt->nsub_prev = cnt->nsub;
 }
 
+// Sub-particle positions of the last step by sub id, for a renderer:
+// (x, y, z, density) of sorted slot q lands at out[sid[sperm[q]]].
+__global__ __launch_bounds__(kB) void k_ts_export(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
+                                                  const float4* __restrict__ svel,
+                                                  const uint32_t* __restrict__ sperm,
+                                                  const uint32_t* __restrict__ sid, float4* __restrict__ out) {
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= cnt->nsub) return;
+    const float4 p = spos[q];
+    out[sid[sperm[q]]] = make_float4(p.x, p.y, p.z, svel[q].w);
+}
+
 }  // namespace
 
 template <class T>
@@ -571,6 +583,25 @@ void twoscale_enqueue(Engine& e) {
                        t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
     hipLaunchKernelGGL(k_ts_prev, dim3(1), dim3(64), 0, st, t.counts);
     e.mark(kStageTwoScaleSubPhysics);
+}
+
+sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view) {
+    TwoScale& t = *e.ts;
+    hipLaunchKernelGGL(k_ts_export, dim3(nblocks(t.cap)), dim3(kB), 0, e.stream, t.counts, t.spos_out, t.svel_out,
+                       t.sperm, t.sid, out);
+    sph_status st = e.check_launch("sub-particle render export");
+    if (st != SPH_OK) return st;
+    TSCounts c;
+    if (hipStreamSynchronize(e.stream) != hipSuccess ||
+        hipMemcpy(&c, t.counts, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+        return e.fail(SPH_DEVICE_ERROR, "count read failed");
+    view->sub_positions = reinterpret_cast<const float*>(out);
+    view->default_ids = t.default_ids;
+    view->sub_ids = t.sid;
+    view->num_default = c.ndefault;
+    view->num_sub_regular = 8 * c.nsplit;
+    view->num_sub_boundary = 8 * c.nboundary;
+    return SPH_OK;
 }
 
 sph_status twoscale_counts(Engine& e, sph_counts* out) {

@@ -11,7 +11,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._native import (Field, SphCounts, SphParams, SphTuning, SPH_MAX_STAGES, as_fptr, check, lib)
+from ._native import (Field, SphCounts, SphParams, SphRenderView, SphTuning, SPH_MAX_STAGES, as_fptr, check,
+                      lib)
 from .scenes import Scene, make_scene
 
 _FLOAT3 = {Field.POSITION, Field.VELOCITY, Field.ACCEL, Field.SUB_POSITION, Field.SUB_VELOCITY,
@@ -147,6 +148,18 @@ class ParticleSimulation:
         p, o = C.c_void_p(), C.c_void_p()
         check(lib.sph_map_positions(self._need(), C.byref(p), C.byref(o)), self._h)
         return p.value, o.value
+
+    def map(self) -> dict:
+        """ParticleSimulation::map (cu:2685-2700): device pointers and counts
+        of the render buffers -- positions by original id, sub-particle
+        positions by sub id (float4 x, y, z, density), the default-state id
+        list and the active sub-particle id list (regular first)."""
+        v = SphRenderView()
+        check(lib.sph_map_render(self._need(), C.byref(v)), self._h)
+        return {f: getattr(v, f) for f, _ in SphRenderView._fields_}
+
+    def unmap(self) -> None:
+        check(lib.sph_unmap_render(self._need()), self._h)
 
     def GetParticleState(self) -> np.ndarray:
         return self.read(Field.STATE)

@@ -217,6 +217,38 @@ sph_status sph_read_field(sph_handle* h, sph_field field, void* dst,
  * the next step or destroy. */
 sph_status sph_map_positions(sph_handle* h, const float** dev_pos4,
                              const uint32_t** dev_orig_id);
+/* Render interop (ParticleSimulation::map / unmap, cu:2685-2707, over the
+ * vertex and index buffers of cu:1926-1932 and 2009-2035): device pointers
+ *   positions      float4 (x, y, z, density) per particle, by original id
+ *                  [num_particles]
+ *   sub_positions  float4 (x, y, z, density) per sub-particle, by
+ *                  sub-particle id 8 * parent + child [8 * num_particles]
+ *   default_ids    original ids of the particles in default state
+ *                  [num_default]
+ *   sub_ids        ids of the active sub-particles, regular (children of
+ *                  split particles) first, then boundary
+ *                  [num_sub_regular + num_sub_boundary]
+ * A renderer draws default_ids out of positions and sub_ids out of
+ * sub_positions (as twoscalestatesub.cpp:96-113 does with GL element
+ * buffers); before the first two-scale step the last three are NULL / 0 and
+ * all particles are drawn from positions.  The reference's vertex buffer has
+ * 3 floats per particle; here 4, the fourth being the density.
+ * sph_map_render fills the buffers on the handle's stream and waits for
+ * them; the pointers stay valid until the next step or destroy. */
+typedef struct sph_render_view {
+    const float* positions;
+    const float* sub_positions;
+    const uint32_t* default_ids;
+    const uint32_t* sub_ids;
+    uint32_t num_particles;
+    uint32_t num_default;
+    uint32_t num_sub_regular;
+    uint32_t num_sub_boundary;
+} sph_render_view;
+sph_status sph_map_render(sph_handle* h, sph_render_view* out);
+/* unmap: ends a map.  The library keeps no mapping state (its buffers are
+ * its own, not a graphics API's), so this only checks the handle. */
+sph_status sph_unmap_render(sph_handle* h);
 sph_status sph_get_counts(sph_handle* h, sph_counts* out);
 size_t sph_num_cells(const sph_handle* h);
 /* GetSizeMemoryGPU (particle_simulation.h:169; not defined in the
