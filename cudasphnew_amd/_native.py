@@ -223,6 +223,8 @@ SIGNATURES = [
     ("sph_map_render", _st, [_vp, C.POINTER(SphRenderView)]),
     ("sph_unmap_render", _st, [_vp]),
     ("sph_get_counts", _st, [_vp, C.POINTER(SphCounts)]),
+    ("sph_save_info_table", _st, [_vp, C.c_char_p]),
+    ("sph_save_particle_info", _st, [_vp, C.c_char_p]),
     ("sph_num_cells", C.c_size_t, [_vp]),
     ("sph_device_bytes", C.c_size_t, [_vp]),
     ("sph_last_error_message", C.c_char_p, [_vp]),

@@ -10,6 +10,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <new>
 
 #include "engine.h"
@@ -783,6 +785,61 @@ sph_status sph_map_render(sph_handle* h, sph_render_view* out) {
 }
 
 sph_status sph_unmap_render(sph_handle* h) { return h ? SPH_OK : SPH_INVALID_ARG; }
+
+sph_status sph_save_info_table(sph_handle* h, const char* filename) {
+    if (!h || !filename) return SPH_INVALID_ARG;
+    Engine& e = h->e;
+    const uint32_t n = e.hbegins[1] - e.hbegins[0];
+    const size_t nc = e.cells_of[0];
+    std::vector<int32_t> ids(n), keys(n), start(nc), end(nc);
+    sph_status st;
+    if ((st = e.read_field(0, SPH_FIELD_SORTED_IDS, ids.data(), n * sizeof(int32_t))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_SORTED_KEYS, keys.data(), n * sizeof(int32_t))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_CELL_START, start.data(), nc * sizeof(int32_t))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_CELL_END, end.data(), nc * sizeof(int32_t))) != SPH_OK)
+        return st;
+    std::ofstream f(filename);
+    if (!f) return e.fail(SPH_INVALID_ARG, std::string("cannot write ") + filename);
+    using std::setw;
+    f << "Number of particles " << n << std::endl;
+    f << setw(8) << "index" << setw(12) << " id" << setw(12) << " hash" << setw(12) << " start" << setw(12) << " end"
+      << std::endl;
+    for (size_t i = 0; i < nc; ++i) {
+        f << setw(8) << i;
+        if (i < n) f << setw(12) << ids[i] << setw(12) << keys[i];
+        else f << setw(12) << "" << setw(12) << "";
+        if (start[i] < 0) f << setw(12) << "";
+        else f << setw(12) << start[i];
+        if (end[i] < 0) f << setw(12) << "" << std::endl;
+        else f << setw(12) << end[i] << std::endl;
+    }
+    return f ? SPH_OK : e.fail(SPH_INVALID_ARG, std::string("write failed: ") + filename);
+}
+
+sph_status sph_save_particle_info(sph_handle* h, const char* filename) {
+    if (!h || !filename) return SPH_INVALID_ARG;
+    Engine& e = h->e;
+    const uint32_t n = e.hbegins[1] - e.hbegins[0];
+    std::vector<float> x(3 * (size_t)n), rho(n), p(n), a(3 * (size_t)n);
+    sph_status st;
+    if ((st = e.read_field(0, SPH_FIELD_POSITION, x.data(), x.size() * sizeof(float))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_DENSITY, rho.data(), rho.size() * sizeof(float))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_PRESSURE, p.data(), p.size() * sizeof(float))) != SPH_OK ||
+        (st = e.read_field(0, SPH_FIELD_ACCEL, a.data(), a.size() * sizeof(float))) != SPH_OK)
+        return st;
+    std::ofstream f(filename);
+    if (!f) return e.fail(SPH_INVALID_ARG, std::string("cannot write ") + filename);
+    const int w = 20;
+    using std::setw;
+    f << setw(w) << "Index" << setw(w) << "X" << setw(w) << "Y" << setw(w) << "Z" << setw(w) << "Density"
+      << setw(w) << "Pressure" << setw(w) << "Acc X" << setw(w) << "Acc Y" << setw(w) << "Acc Z" << std::endl;
+    for (uint32_t i = 0; i < n; ++i) {
+        f << setw(w) << i << setw(w) << x[3 * (size_t)i] << setw(w) << x[3 * (size_t)i + 1] << setw(w)
+          << x[3 * (size_t)i + 2] << setw(w) << rho[i] << setw(w) << p[i] << setw(w) << a[3 * (size_t)i] << setw(w)
+          << a[3 * (size_t)i + 1] << setw(w) << a[3 * (size_t)i + 2] << std::endl;
+    }
+    return f ? SPH_OK : e.fail(SPH_INVALID_ARG, std::string("write failed: ") + filename);
+}
 
 sph_status sph_get_counts(sph_handle* h, sph_counts* out) {
     if (!h || !out) return SPH_INVALID_ARG;

@@ -8,6 +8,7 @@ reference throws std::runtime_error from CUDA_SAFE_CALL (util.h:24-39).
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -195,34 +196,12 @@ class ParticleSimulation:
 
     # -- debug dumps (SaveInfoTable / SaveParticleInfo, cu:2709-2852) -------
     def SaveInfoTable(self, filename: str) -> None:
-        ids = self.read(Field.SORTED_IDS)
-        keys = self.read(Field.SORTED_KEYS)
-        start = self.read(Field.CELL_START)
-        end = self.read(Field.CELL_END)
-        n = ids.shape[0]
-        with open(filename, "w") as f:
-            f.write(f"Number of particles {n}\n")
-            f.write(f"{'index':>8}{' id':>12}{' hash':>12}{' start':>12}{' end':>12}\n")
-            for i in range(start.shape[0]):
-                row = f"{i:>8}"
-                row += f"{ids[i]:>12}{keys[i]:>12}" if i < n else f"{'':>12}{'':>12}"
-                row += f"{'':>12}" if start[i] < 0 else f"{start[i]:>12}"
-                row += f"{'':>12}" if end[i] < 0 else f"{end[i]:>12}"
-                f.write(row + "\n")
+        """ParticleSimulation::SaveInfoTable (cu:2709-2780), same format."""
+        check(lib.sph_save_info_table(self._need(), os.fsencode(filename)), self._h)
 
     def SaveParticleInfo(self, filename: str) -> None:
-        x = self.positions()
-        rho = self.read(Field.DENSITY)
-        p = self.read(Field.PRESSURE)
-        a = self.read(Field.ACCEL)
-        w = 20
-        cols = ["Index", "X", "Y", "Z", "Density", "Pressure", "Acc X", "Acc Y", "Acc Z"]
-        with open(filename, "w") as f:
-            f.write("".join(f"{c:>{w}}" for c in cols) + "\n")
-            for i in range(x.shape[0]):
-                vals = [i, *x[i], rho[i], p[i], *a[i]]
-                f.write("".join(f"{v:>{w}.6g}" if isinstance(v, (float, np.floating)) else f"{v:>{w}}"
-                                for v in vals) + "\n")
+        """ParticleSimulation::SaveParticleInfo (cu:2782-2852), same format."""
+        check(lib.sph_save_particle_info(self._need(), os.fsencode(filename)), self._h)
 
     # -- timing ----------------------------------------------------------------
     def enable_timing(self, on: bool = True) -> None:

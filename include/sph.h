@@ -250,6 +250,15 @@ sph_status sph_map_render(sph_handle* h, sph_render_view* out);
  * its own, not a graphics API's), so this only checks the handle. */
 sph_status sph_unmap_render(sph_handle* h);
 sph_status sph_get_counts(sph_handle* h, sph_counts* out);
+/* Debug text dumps in the reference's column formats (iostream setw):
+ * SaveInfoTable (cu:2709-2780): "Number of particles N", header, then one
+ * row per cell index i: i, and for i < N the sorted id and hash of slot i,
+ * then the cell's start and end (blank when empty).  SaveParticleInfo
+ * (cu:2782-2852): header and one row per particle id: position, density,
+ * pressure, acceleration, 20 characters per column, default float format.
+ * Scene 0 of the handle; host buffers, a device-to-host copy of the fields. */
+sph_status sph_save_info_table(sph_handle* h, const char* filename);
+sph_status sph_save_particle_info(sph_handle* h, const char* filename);
 size_t sph_num_cells(const sph_handle* h);
 /* GetSizeMemoryGPU (particle_simulation.h:169; not defined in the
  * reference): device bytes owned by the handle. */
