@@ -224,6 +224,7 @@ SIGNATURES = [
     ("sph_unmap_render", _st, [_vp]),
     ("sph_get_counts", _st, [_vp, C.POINTER(SphCounts)]),
     ("sph_save_info_table", _st, [_vp, C.c_char_p]),
+    ("sph_extract_surface", _st, [_vp, C.c_void_p, C.POINTER(C.c_uint32)]),
     ("sph_save_particle_info", _st, [_vp, C.c_char_p]),
     ("sph_num_cells", C.c_size_t, [_vp]),
     ("sph_device_bytes", C.c_size_t, [_vp]),

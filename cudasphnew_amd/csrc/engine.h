@@ -100,6 +100,7 @@ struct Engine {
     // render interop buffers, allocated on the first map
     float4* render_pos = nullptr;   // [N] by original id
     float4* render_sub = nullptr;   // [8N] by sub-particle id
+    uint8_t* surface = nullptr;     // [N] surface flags by original id, on demand
 
     ~Engine();
     void drop_graph();
@@ -118,6 +119,7 @@ struct Engine {
     sph_status read_field(int scene, sph_field f, void* dst, size_t bytes_out);
     sph_status upload_params(int scene, const sph_params& p);
     sph_status map_render(sph_render_view* out);
+    sph_status extract_surface(uint8_t* flags, uint32_t* count);
 };
 
 DevScene make_dev_scene(const sph_params& P, const sph_boundary& B);

@@ -52,6 +52,9 @@ void launch_wall_modes(hipStream_t st, const DevScene& S, const float4* grid4, u
 // the hit-mask pair is launched instead and the force's lanes value selects
 // its per-thread hit-list capacity (32, 48 or 64).
 void launch_density(hipStream_t st, const StepBuffers& b, int lanes);
+// Surface flag per original id from the sorted positions and cell ranges of
+// the last step (k_surface).
+void launch_surface(hipStream_t st, const StepBuffers& b, float cm_thresh, float n_thresh, uint8_t* out);
 // out[oid[i]] = (pos[i].xyz, vel[i].w) for i < n: positions by original id.
 void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, const float4* vel, const uint32_t* oid,
                              float4* out);

@@ -797,6 +797,44 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) 
                  status, nr);
 }
 
+// Surface particles (the reference's declared extractSurfaceParticles, on
+// the positions of the last step's force pass): neighbour count including
+// the particle itself and the offset of the neighbourhood's centre of mass,
+// accumulated as differences x_j - x_i (no cancellation against |x|).
+__global__ __launch_bounds__(kBlock) void k_surface(const float4* __restrict__ pos, const int2* __restrict__ range,
+                                                    uint32_t n, const DevScene* __restrict__ scenes,
+                                                    const uint32_t* __restrict__ begins, int nscenes,
+                                                    const uint32_t* __restrict__ oid, float cm_thresh,
+                                                    float n_thresh, uint8_t* __restrict__ out) {
+    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const float4 xi = pos[s];
+    Rows R = neighbour_rows(S, xi);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cnt = 1.0f;
+    for (int dk = 0; dk < 3; ++dk)
+        for (int dj = 0; dj < 3; ++dj) {
+            int b, e;
+            row_slots(range, S, R, dk, dj, xi, b, e);
+            for (int q = b; q < e; ++q) {
+                const float4 xj = pos[q];
+                const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                if (r2 <= S.h2 && r2 > 0.0f) {
+                    sx += dx;
+                    sy += dy;
+                    sz += dz;
+                    cnt += 1.0f;
+                }
+            }
+        }
+    sx /= cnt;
+    sy /= cnt;
+    sz /= cnt;
+    const float d = sqrtf(sx * sx + sy * sy + sz * sz);
+    out[oid[s]] = (cnt < n_thresh || d > cm_thresh * S.h) ? 1 : 0;
+}
+
 // Boundary source of the cells of one scene from its grid: a cell whose
 // particles can only reach texels holding the far-field values (distance
 // -r0, no wall density or viscosity) takes no wall term and no lookup --
@@ -882,6 +920,12 @@ void launch_density(hipStream_t st, const StepBuffers& b, int lanes) {
 #define SPH_FORCE_ARGS_M                                                                                            \
     b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
         b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.masks, b.nr
+
+void launch_surface(hipStream_t st, const StepBuffers& b, float cm_thresh, float n_thresh, uint8_t* out) {
+    if (b.n == 0) return;
+    hipLaunchKernelGGL(k_surface, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.pos_b, b.range, b.n, b.scenes,
+                       b.begins, b.nscenes, b.oid_b, cm_thresh, n_thresh, out);
+}
 
 void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, const float4* vel, const uint32_t* oid,
                              float4* out) {

@@ -111,7 +111,7 @@ Engine::~Engine() {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
                     perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode,
-                    render_pos, render_sub};
+                    render_pos, render_sub, surface};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     sorter.release();
@@ -654,6 +654,27 @@ sph_status Engine::read_field(int s, sph_field f, void* dst, size_t nbytes) {
     }
 }
 
+sph_status Engine::extract_surface(uint8_t* flags, uint32_t* count) {
+    if (nscenes != 1) return fail(SPH_INVALID_ARG, "surface extraction is per handle (one scene)");
+    if (!cells_valid) return fail(SPH_INVALID_ARG, "surface extraction needs the state of one step: advance once first");
+    if (!surface) {
+        sph_status st = alloc(&surface, N);
+        if (st != SPH_OK) return st;
+        bytes += N;
+    }
+    launch_surface(stream, buffers(), params[0].cm_distance_thresh, params[0].n_part_thresh, surface);
+    sph_status st = check_launch("surface extraction");
+    if (st != SPH_OK) return st;
+    if (hipStreamSynchronize(stream) != hipSuccess || hipMemcpy(flags, surface, N, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "surface extraction failed");
+    if (count) {
+        uint32_t c = 0;
+        for (uint32_t i = 0; i < N; ++i) c += flags[i];
+        *count = c;
+    }
+    return SPH_OK;
+}
+
 // Render interop: the storage arrays hold the last step's state in cell
 // order; export the positions by original id (and the sub-particles by sub
 // id) into buffers a renderer can index with the id lists.
@@ -785,6 +806,11 @@ sph_status sph_map_render(sph_handle* h, sph_render_view* out) {
 }
 
 sph_status sph_unmap_render(sph_handle* h) { return h ? SPH_OK : SPH_INVALID_ARG; }
+
+sph_status sph_extract_surface(sph_handle* h, uint8_t* is_surface, uint32_t* num_surface) {
+    if (!h || !is_surface) return SPH_INVALID_ARG;
+    return h->e.extract_surface(is_surface, num_surface);
+}
 
 sph_status sph_save_info_table(sph_handle* h, const char* filename) {
     if (!h || !filename) return SPH_INVALID_ARG;

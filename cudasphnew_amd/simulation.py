@@ -159,6 +159,15 @@ class ParticleSimulation:
         check(lib.sph_map_render(self._need(), C.byref(v)), self._h)
         return {f: getattr(v, f) for f, _ in SphRenderView._fields_}
 
+    def extract_surface(self):
+        """extractSurfaceParticles (declared at particle_simulation.h:224,
+        defined here; see sph_extract_surface): (flags by original id,
+        uint8 0/1; number flagged)."""
+        flags = np.zeros(self.scene.n, dtype=np.uint8)
+        cnt = C.c_uint32()
+        check(lib.sph_extract_surface(self._need(), flags.ctypes.data, C.byref(cnt)), self._h)
+        return flags, int(cnt.value)
+
     def unmap(self) -> None:
         check(lib.sph_unmap_render(self._need()), self._h)
 

@@ -258,6 +258,19 @@ sph_status sph_get_counts(sph_handle* h, sph_counts* out);
  * pressure, acceleration, 20 characters per column, default float format.
  * Scene 0 of the handle; host buffers, a device-to-host copy of the fields. */
 sph_status sph_save_info_table(sph_handle* h, const char* filename);
+/* extractSurfaceParticles: declared by the reference (particle_simulation.h:224)
+ * but never defined; its inputs are -- the thresholds cmDistanceThresh and
+ * nPartTresh (h:103-104, set to 0.5 and 20 by Example01, cu:1811-1812) and
+ * the neighbourhood centre-of-mass sums of compute_particle_acceleration_ifsurf
+ * (cu:370-380, 1550-1553: the particle itself plus every neighbour with
+ * 0 < r <= h).  Definition used here, on the positions the last step's
+ * forces were computed from (the state the reference's kernel would see):
+ *   n_i = 1 + #{j : 0 < |x_j - x_i| <= h},
+ *   d_i = | sum_j (x_j - x_i) | / n_i   (centre of mass minus x_i),
+ *   surface  <=>  n_i < nPartTresh  or  d_i > cmDistanceThresh * h.
+ * is_surface: one byte per particle, by original id (0/1); num_surface:
+ * count of flagged particles (may be NULL).  Needs one step first. */
+sph_status sph_extract_surface(sph_handle* h, uint8_t* is_surface, uint32_t* num_surface);
 sph_status sph_save_particle_info(sph_handle* h, const char* filename);
 size_t sph_num_cells(const sph_handle* h);
 /* GetSizeMemoryGPU (particle_simulation.h:169; not defined in the
