@@ -93,6 +93,16 @@ def copy_probe(nbytes: int) -> None:
     log(f"copy probe: {nbytes} bytes read, {nbytes} bytes written")
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, seconds: float, threads: int):
     from oracle import oracle as O
     sim = O.OracleSim(scene, real=1, threads=threads)
@@ -105,6 +115,7 @@ def cpu_baseline(scene, seconds: float, threads: int):
         "value": scene.n * steps / total,
         "unit": "particle-updates/s",
         "cores": threads,
+        "cpu": f"{cpu_model()} ({os.cpu_count()} logical CPUs visible)",
         "kind": "port",
         "sample": f"{steps} steps of the same {scene.n}-particle workload after 1 warm-up step, "
                   f"FP32 C++ transcription of the reference step (oracle/, OpenMP, {threads} threads), "
