@@ -46,6 +46,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 SOURCES = [
     ("scene.cpp", "host"),
     ("radix_sort.hip", "device"),
+    ("cell_sort.hip", "device"),
     ("kernels.hip", "device"),
     ("twoscale.hip", "device"),
     ("sim.hip", "device"),

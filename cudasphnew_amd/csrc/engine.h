@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.h"
+#include "cell_sort.h"
 #include "kernels.h"
 #include "radix_sort.h"
 
@@ -64,6 +65,11 @@ struct Engine {
     uint32_t* status = nullptr;
     uint32_t* n_dev = nullptr;
     RadixSort sorter;
+    // counting sort over the cell table, used when the key space is at most
+    // kCellSortRatio cells per particle (every workload here: 2.5-9); the
+    // radix sort otherwise
+    CellSort csort;
+    bool use_cell_sort = false;
     size_t bytes = 0;
 
     bool keys_valid = false;

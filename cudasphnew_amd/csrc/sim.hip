@@ -115,6 +115,7 @@ Engine::~Engine() {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     sorter.release();
+    csort.release();
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -218,11 +219,20 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
     SPH_TRY(alloc(&range, total_cells));
     SPH_TRY(alloc(&status, 1));
     SPH_TRY(alloc(&n_dev, 1));
-    if (sorter.init(N) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(SPH_OUT_OF_MEMORY, "radix sort buffers");
+    use_cell_sort = (uint64_t)total_cells <= (uint64_t)kCellSortRatio * N + 65536;
+    if (use_cell_sort) {
+        if (csort.init(total_cells, N) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SPH_OUT_OF_MEMORY, "cell sort buffers");
+        }
+        bytes += csort.bytes;
+    } else {
+        if (sorter.init(N) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(SPH_OUT_OF_MEMORY, "radix sort buffers");
+        }
+        bytes += sorter.bytes();
     }
-    bytes += sorter.bytes();
 
     // host staging, then upload
     std::vector<float4> hp(N), hv(N), g4;
@@ -346,10 +356,16 @@ void Engine::enqueue_step(bool two_scale) {
     mark(kStageHash);
     if (cells_valid) launch_cells_clear(stream, b);
     mark(kStageClear);
-    sorter.sort(stream, key, nullptr, skey, perm, n_dev, key_bits);
-    mark(kStageSort);
-    launch_cells(stream, b);
-    mark(kStageCells);
+    if (use_cell_sort) {
+        // the counting sort writes the cell ranges itself
+        csort.sort(stream, key, N, skey, perm, range);
+        mark(kStageSort);
+    } else {
+        sorter.sort(stream, key, nullptr, skey, perm, n_dev, key_bits);
+        mark(kStageSort);
+        launch_cells(stream, b);
+        mark(kStageCells);
+    }
     launch_reorder(stream, b);
     mark(kStageReorder);
     launch_density(stream, b, tuning.density_lanes);
