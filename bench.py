@@ -197,6 +197,7 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
     rep = sim.attach_neural(net)
     sim.Advance(max(a.warmup, 1))
     tun = sim.autotune(reps=3)
+    sim.use_graph(not a.no_graph)
     sim.Advance(2)
     ms = sim.time_steps(a.steps) / a.steps
     sim.sync()
@@ -317,6 +318,7 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprof byte counters)")
     ap.add_argument("--no-forms", action="store_true", help="skip the learned-boundary and batched forms")
+    ap.add_argument("--no-graph", action="store_true", help="time the 1M steps eagerly instead of as graph replays")
     ap.add_argument("--train-profile", type=int, default=0,
                     help="train the 1M scene's network for this many steps (batch 65536), report and exit")
     ap.add_argument("--train-steps", type=int, default=20000, help="training steps of the 1M scene's network")
@@ -358,11 +360,19 @@ def main(argv=None) -> int:
         sim.sync()
         log(f"profile: {a.profile_steps} steps done")
         return 0
-    sim.enable_timing(True)
+    # the timed steps replay the captured step graph; the per-stage
+    # breakdown comes from a second run of the same number of steps with an
+    # event around every stage (launched eagerly: events split the graph)
+    sim.use_graph(not a.no_graph)
+    sim.Advance(2)
+    sim.sync()
     ms = sim.time_steps(a.steps)
+    sim.enable_timing(True)
+    ms_staged = sim.time_steps(a.steps)
     stages = sim.stage_times()
     sim.enable_timing(False)
-    log(f"timed {a.steps} steps: {ms:.2f} ms; stages (ms/step): "
+    log(f"timed {a.steps} steps: {ms:.2f} ms ({'graph' if not a.no_graph else 'eager'}); "
+        f"with stage events {ms_staged:.2f} ms; stages (ms/step): "
         + ", ".join(f"{k}={v:.4f}" for k, v in stages.items() if v > 0))
 
     n = sc.n

@@ -221,6 +221,7 @@ SIGNATURES = [
     ("sph_read_field", _st, [_vp, C.c_int, _vp, C.c_size_t]),
     ("sph_map_positions", _st, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     ("sph_map_render", _st, [_vp, C.POINTER(SphRenderView)]),
+    ("sph_use_graph", _st, [_vp, C.c_int32]),
     ("sph_unmap_render", _st, [_vp]),
     ("sph_get_counts", _st, [_vp, C.POINTER(SphCounts)]),
     ("sph_save_info_table", _st, [_vp, C.c_char_p]),

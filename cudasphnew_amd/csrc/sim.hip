@@ -950,6 +950,13 @@ static sph_status time_engine(Engine& e, int32_t n, bool two_scale, float* ms) {
     return e.sync();
 }
 
+sph_status sph_use_graph(sph_handle* h, int32_t enable) {
+    if (!h) return SPH_INVALID_ARG;
+    h->e.use_graph = enable != 0;
+    if (!h->e.use_graph) h->e.drop_graph();
+    return SPH_OK;
+}
+
 sph_status sph_time_steps(sph_handle* h, int32_t n, int32_t two_scale, float* ms) {
     return h ? time_engine(h->e, n, two_scale != 0, ms) : SPH_INVALID_ARG;
 }

@@ -222,6 +222,10 @@ class ParticleSimulation:
         check(lib.sph_stage_times(self._need(), ms, C.byref(cnt)), self._h)
         return {lib.sph_stage_name(i).decode(): float(ms[i]) for i in range(cnt.value)}
 
+    def use_graph(self, enable: bool = True) -> None:
+        """Replay the steady-state plain step from a captured hipGraph."""
+        check(lib.sph_use_graph(self._need(), 1 if enable else 0), self._h)
+
     def time_steps(self, n: int, two_scale: bool = False) -> float:
         ms = C.c_float()
         check(lib.sph_time_steps(self._need(), int(n), 1 if two_scale else 0, C.byref(ms)), self._h)

@@ -292,6 +292,12 @@ sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count);
 const char* sph_stage_name(int32_t stage);
 /* Device time of n steps measured with HIP events recorded on the handle's
  * stream around the steps (no host work inside the timed region). */
+/* Replays the steady-state plain step from a captured hipGraph (captured on
+ * the first eligible step, re-captured after any change of parameters,
+ * kernel variants or learned boundary); steps with per-stage timing on and
+ * two-scale steps run eagerly.  No reference counterpart (the reference
+ * launches every kernel of every step); as sph_batch_use_graph. */
+sph_status sph_use_graph(sph_handle* h, int32_t enable);
 sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
                           float* ms);
 
