@@ -241,6 +241,26 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
             entry["cpu_oracle"] = "FP64 restatement (oracle/mesh_oracle.cpp), OpenMP"
         forms[f"boundary_map_{name}"] = entry
         log(f"boundary map {name}: {entry['dims']} in {entry['ms']} ms")
+    # two-scale step (AdvanceTwoScale): 262,144 coarse particles with the
+    # split region around the sphere
+    sc2 = S.make_scene("two_scale")
+    sim2 = S.ParticleSimulation(sc2).Init()
+    sim2.Advance(1)
+    sim2.AdvanceTwoScale(max(a.warmup, 1))
+    sim2.sync()
+    sim2.enable_timing(True)
+    ms2 = sim2.time_steps(a.steps, two_scale=True) / a.steps
+    st2 = sim2.stage_times()
+    sim2.enable_timing(False)
+    c2 = sim2.counts()
+    forms["two_scale_262k"] = {
+        "value": round(sc2.n / (ms2 / 1e3), 1), "ms_per_step": round(ms2, 5), "particles": sc2.n,
+        "sub_particles": int(c2.num_sub), "split": int(c2.num_split), "boundary": int(c2.num_boundary),
+        "stage_ms": {k: round(v, 5) for k, v in st2.items() if v > 0},
+        "note": "coarse particle-updates/s of AdvanceTwoScale (sub-particle pass included), eager with stage events",
+    }
+    log(f"two-scale 262K: {ms2:.4f} ms/step, {c2.num_sub} sub-particles")
+    del sim2
     # 64 scenes
     t0 = time.time()
     scenes = [S.make_scene("batch", index=i, seed=1) for i in range(64)]

@@ -30,6 +30,14 @@ struct CellSort {
     // end) for every occupied cell k.  range entries of other cells are not
     // touched (the caller invalidates the previous step's cells).
     void sort(hipStream_t st, const uint32_t* keys, uint32_t n, uint32_t* skey, uint32_t* perm, int2* range);
+    // The same with the count read on the device (n_dev <= capacity), so
+    // that the call can be captured in a graph; grids sized for capacity.
+    void sort(hipStream_t st, const uint32_t* keys, const uint32_t* n_dev, uint32_t capacity, uint32_t* skey,
+              uint32_t* perm, int2* range);
+
+  private:
+    void launch(hipStream_t st, const uint32_t* keys, uint32_t n, const uint32_t* n_dev, uint32_t grid_n,
+                uint32_t* skey, uint32_t* perm, int2* range);
 };
 
 }  // namespace sph

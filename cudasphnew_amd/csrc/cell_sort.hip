@@ -31,9 +31,12 @@ constexpr int kCsBlockCells = kCsThreads * kCsPerThread;   // 4096
 // each run adds the run's length with one atomic and hands the old count to
 // the run, whose lanes rank themselves by their offset in it.  Equal keys
 // in separate runs take separate atomics, which is still exact.
-__global__ __launch_bounds__(kCsThreads) void k_cs_count(const uint32_t* __restrict__ keys, uint32_t n,
+__global__ __launch_bounds__(kCsThreads) void k_cs_count(const uint32_t* __restrict__ keys, uint32_t n_host,
+                                                         const uint32_t* __restrict__ n_dev,
                                                          uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
+    const uint32_t n = n_dev ? *n_dev : n_host;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((i & ~63u) >= n) return;   // the whole wavefront lies past the end
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
     const uint32_t k = valid ? keys[i] : 0xFFFFFFFFu;
@@ -138,10 +141,12 @@ __global__ __launch_bounds__(kCsThreads) void k_cs_ranges(uint32_t* __restrict__
     }
 }
 
-__global__ __launch_bounds__(kCsThreads) void k_cs_scatter(const uint32_t* __restrict__ keys, uint32_t n,
+__global__ __launch_bounds__(kCsThreads) void k_cs_scatter(const uint32_t* __restrict__ keys, uint32_t n_host,
+                                                           const uint32_t* __restrict__ n_dev,
                                                            const uint32_t* __restrict__ rank,
                                                            const int2* __restrict__ range,
                                                            uint32_t* __restrict__ skey, uint32_t* __restrict__ perm) {
+    const uint32_t n = n_dev ? *n_dev : n_host;
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t k = keys[i];
@@ -181,10 +186,22 @@ void CellSort::release() {
 void CellSort::sort(hipStream_t st, const uint32_t* keys, uint32_t n, uint32_t* skey, uint32_t* perm,
                     int2* range) {
     if (n == 0) return;
-    hipLaunchKernelGGL(k_cs_count, dim3(grid_for(n)), dim3(kCsThreads), 0, st, keys, n, cnt, rank);
+    launch(st, keys, n, nullptr, n, skey, perm, range);
+}
+
+void CellSort::sort(hipStream_t st, const uint32_t* keys, const uint32_t* n_dev, uint32_t capacity, uint32_t* skey,
+                    uint32_t* perm, int2* range) {
+    if (capacity == 0) return;
+    launch(st, keys, 0, n_dev, capacity, skey, perm, range);
+}
+
+void CellSort::launch(hipStream_t st, const uint32_t* keys, uint32_t n, const uint32_t* n_dev, uint32_t grid_n,
+                      uint32_t* skey, uint32_t* perm, int2* range) {
+    hipLaunchKernelGGL(k_cs_count, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, cnt, rank);
     hipLaunchKernelGGL(k_cs_reduce, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum);
     hipLaunchKernelGGL(k_cs_ranges, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum, range);
-    hipLaunchKernelGGL(k_cs_scatter, dim3(grid_for(n)), dim3(kCsThreads), 0, st, keys, n, rank, range, skey, perm);
+    hipLaunchKernelGGL(k_cs_scatter, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, rank, range,
+                       skey, perm);
 }
 
 }  // namespace sph

@@ -14,9 +14,10 @@
 //   seed     initialize_sub_particles (cu:1023-1090) + project_quantities
 //            (cu:570-622) for the boundary sub-particles.
 //   sort     sub hash + sort + cell ranges on the sub grid (cu:121-153,
-//            2327-2389); the radix sort is stable over the sub list, whose
-//            order (split parents then boundary parents, by id, children in
-//            order) is the reference's tie order.
+//            2327-2389): the counting sort of cell_sort.hip, then each cell
+//            ranked back into list order (split parents then boundary
+//            parents, by id, children in order: the reference's tie order);
+//            the stable radix sort for sub grids too large for it.
 //   physics  sub density (split subs), sub acceleration (split subs), sub
 //            integration + collision (all subs) (cu:259-313, 477-568,
 //            667-696, 836-930).
@@ -50,6 +51,12 @@ struct TwoScale {
     float4 *spos_out = nullptr, *svel_out = nullptr, *sacc = nullptr;  // sorted, after the step
     int2* srange = nullptr;
     RadixSort sorter;
+    // counting sort over the sub grid (used when it has at most
+    // kCellSortRatio cells per sub-particle slot); its order inside a cell is
+    // replaced by list order in k_ts_gather_ranked
+    CellSort csort;
+    bool use_cell_sort = false;
+    uint32_t* sperm_raw = nullptr;
     size_t bytes = 0;
     bool sub_cells_valid = false;
 };
@@ -90,18 +97,41 @@ __global__ __launch_bounds__(kB) void k_ts_boundary(const float4* __restrict__ p
     int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
     int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
     int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+    // split particles lie in the split box: a particle farther than h from
+    // the box has none within h (the 1e-5 margin covers the rounding)
+    const float bx = fmaxf(fmaxf(S.split_lo[0] - xi.x, xi.x - S.split_hi[0]), 0.0f);
+    const float by = fmaxf(fmaxf(S.split_lo[1] - xi.y, xi.y - S.split_hi[1]), 0.0f);
+    const float bz = fmaxf(fmaxf(S.split_lo[2] - xi.z, xi.z - S.split_hi[2]), 0.0f);
     bool hit = false;
+    if ((bx * bx + by * by + bz * bz) * 0.99999f > h2) hz = lz - 1;   // no row to visit
+    // the cells of an x-row have consecutive keys, so their particles are
+    // one run of slots: 9 runs, the 3 range loads of a row issued together
     for (int k = lz; k <= hz && !hit; ++k)
-        for (int j = ly; j <= hy && !hit; ++j)
-            for (int i = lx; i <= hx && !hit; ++i) {
-                int2 r = range[S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j) + i)];
-                for (int q = r.x; q < r.y; ++q) {
-                    if (!(state[q] & 1)) continue;
-                    float4 xj = pos[q];
-                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    if (dx * dx + dy * dy + dz * dz <= h2) { hit = true; break; }
+        for (int j = ly; j <= hy && !hit; ++j) {
+            const uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
+            int b = 0x7fffffff, e = -1;
+            for (int i = lx; i <= hx; ++i) {
+                const int2 r = range[rowbase + (uint32_t)i];
+                if (r.x >= 0) {
+                    b = min(b, r.x);
+                    e = max(e, r.y);
                 }
             }
+            // split flags eight at a time: the loads are issued together
+            // instead of one latency per candidate
+            for (int q0 = b; q0 < e && !hit; q0 += 8) {
+                uint8_t f[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) f[u] = q0 + u < e ? state[q0 + u] : (uint8_t)0;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (!(f[u] & 1)) continue;
+                    const float4 xj = pos[q0 + u];
+                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    if (dx * dx + dy * dy + dz * dz <= h2) hit = true;
+                }
+            }
+        }
     uint8_t st = state[s];
     if (hit) st |= 2;
     state_fin[s] = st;
@@ -331,6 +361,29 @@ __global__ __launch_bounds__(kB) void k_ts_cells_gather(const TSCounts* __restri
     svel_s[t] = svel[src];
 }
 
+// After the counting sort: each sub cell's slots in list order (the
+// reference's stable tie order: the rank of the list index among the cell's
+// slots), then the gather into sorted order.  The cell ranges come from the
+// sort.
+__global__ __launch_bounds__(kB) void k_ts_gather_ranked(const TSCounts* __restrict__ cnt,
+                                                         const uint32_t* __restrict__ sskey,
+                                                         const uint32_t* __restrict__ sperm_raw,
+                                                         const int2* __restrict__ srange,
+                                                         const float4* __restrict__ spos,
+                                                         const float4* __restrict__ svel, uint32_t* __restrict__ sperm,
+                                                         float4* __restrict__ spos_s, float4* __restrict__ svel_s) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->nsub) return;
+    const int2 r = srange[sskey[t]];
+    const uint32_t src = sperm_raw[t];
+    uint32_t rank = 0;
+    for (int q = r.x; q < r.y; ++q) rank += sperm_raw[q] < src ? 1u : 0u;
+    const uint32_t dst = (uint32_t)r.x + rank;
+    sperm[dst] = src;
+    spos_s[dst] = spos[src];
+    svel_s[dst] = svel[src];
+}
+
 __device__ __forceinline__ void sub_range(const DevScene& S, float4 x, float h, int& lx, int& hx, int& ly, int& hy,
                                           int& lz, int& hz) {
     lx = range_coord(x.x, -h, S.ox, S.cell_sub, S.snx); hx = range_coord(x.x, h, S.ox, S.cell_sub, S.snx);
@@ -521,14 +574,18 @@ sph_status twoscale_create(Engine& e) {
               ts_alloc(*t, &t->sskey, t->cap) == hipSuccess && ts_alloc(*t, &t->sperm, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->spos_s, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_s, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->spos_out, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_out, t->cap) == hipSuccess &&
-              ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess &&
-              t->sorter.init(t->cap) == hipSuccess;
+              ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess;
+    t->use_cell_sort = (uint64_t)t->sub_cells <= (uint64_t)kCellSortRatio * t->cap;
+    if (ok && t->use_cell_sort)
+        ok = ts_alloc(*t, &t->sperm_raw, t->cap) == hipSuccess && t->csort.init(t->sub_cells, t->cap) == hipSuccess;
+    else if (ok)
+        ok = t->sorter.init(t->cap) == hipSuccess;
     e.ts = t;   // owned by the engine from here on (freed by twoscale_destroy)
     if (!ok) {
         (void)hipGetLastError();
         return e.fail(SPH_OUT_OF_MEMORY, "two-scale buffers");
     }
-    t->bytes += t->sorter.bytes();
+    t->bytes += t->use_cell_sort ? t->csort.bytes : t->sorter.bytes();
     if (hipMemset(t->srange, 0xFF, (size_t)t->sub_cells * sizeof(int2)) != hipSuccess ||
         hipMemset(t->counts, 0, sizeof(TSCounts)) != hipSuccess ||
         hipMemset(t->state_fin, 0, e.N) != hipSuccess || hipDeviceSynchronize() != hipSuccess)
@@ -542,10 +599,11 @@ void twoscale_destroy(Engine& e) {
     void* ptrs[] = {t->state_fin, t->cls,   t->split_ids, t->boundary_ids, t->default_ids, t->blk,
                     t->counts,    t->spos,  t->svel,      t->sid,          t->skey,        t->sskey,
                     t->sperm,     t->spos_s, t->svel_s,   t->spos_out,     t->svel_out,    t->sacc,
-                    t->srange};
+                    t->srange,    t->sperm_raw};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t->sorter.release();
+    t->csort.release();
     delete t;
     e.ts = nullptr;
 }
@@ -572,9 +630,15 @@ void twoscale_enqueue(Engine& e) {
     e.mark(kStageTwoScaleSeed);
     hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
                        e.status);
-    t.sorter.sort(st, t.skey, nullptr, t.sskey, t.sperm, &t.counts->nsub, t.sub_bits);
-    hipLaunchKernelGGL(k_ts_cells_gather, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm, t.spos,
-                       t.svel, t.srange, t.spos_s, t.svel_s);
+    if (t.use_cell_sort) {
+        t.csort.sort(st, t.skey, &t.counts->nsub, t.cap, t.sskey, t.sperm_raw, t.srange);
+        hipLaunchKernelGGL(k_ts_gather_ranked, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm_raw,
+                           t.srange, t.spos, t.svel, t.sperm, t.spos_s, t.svel_s);
+    } else {
+        t.sorter.sort(st, t.skey, nullptr, t.sskey, t.sperm, &t.counts->nsub, t.sub_bits);
+        hipLaunchKernelGGL(k_ts_cells_gather, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm,
+                           t.spos, t.svel, t.srange, t.spos_s, t.svel_s);
+    }
     t.sub_cells_valid = true;
     e.mark(kStageTwoScaleSubSort);
     hipLaunchKernelGGL(k_ts_density, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
