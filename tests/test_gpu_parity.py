@@ -150,6 +150,31 @@ def test_ragged_sizes(dam16k, n):
     _check_accel(sim.read(Field.ACCEL), orc)
 
 
+def test_sparse_scene_takes_the_radix_sort():
+    """20,000 particles of the 1M lattice in its 8,923,200-cell grid: more
+    than 16 cells per particle, so the step sorts with the LSD radix sort and
+    the separate cell-range pass instead of the counting sort."""
+    big = S.make_scene("dam1m")
+    rng = np.random.default_rng(5)
+    pos = big.positions[np.sort(rng.choice(big.n, 20000, replace=False))].copy()
+    sc = big.with_positions(pos)
+    cells = int(np.prod([sc.params.grid_dim[a] for a in range(3)]))
+    assert cells > 16 * sc.n + 65536
+    sim = _sim(sc)
+    sim.Advance(1)
+    sim.sync()
+    _check_sort(sc.params, pos, sim)
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1)
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
+    _check_accel(sim.read(Field.ACCEL), orc)
+    sim.Advance(20)
+    pos = sim.positions()
+    sim.Advance(1)
+    sim.sync()
+    _check_sort(sc.params, pos, sim)
+
+
 def test_single_particle_free_fall(dam16k):
     """One particle far from the walls has no neighbour but itself: symplectic
     Euler from rest gives y_n = y0 - g dt^2 n(n+1)/2 exactly."""
