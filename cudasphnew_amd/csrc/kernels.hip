@@ -10,9 +10,10 @@
 // the sorted id list instead (particle_simulation.cu:206,327).
 //
 // Per step:  [hash, first step only] -> clear last step's cells ->
-//   radix sort (key, storage slot) -> cell ranges -> reorder (+ tie fix-up
-//   to original-id order) -> density -> force + integrate + collision +
-//   next key.
+//   sort by key (counting sort over the cell table, which also writes the
+//   cell ranges; or radix sort + cell ranges) -> reorder (+ tie fix-up to
+//   original-id order) -> density -> force + integrate + collision + next
+//   key.
 #include "kernels.h"
 
 namespace sph {
@@ -60,10 +61,10 @@ __global__ __launch_bounds__(kBlock) void k_cells(const uint32_t* __restrict__ s
     if (s == n - 1 || skey[s + 1] != k) r[2 * (size_t)k + 1] = (int)(s + 1);
 }
 
-// Gather storage -> sorted order.  The radix sort is stable with respect to
-// the storage order; the reference sorts (hash, original id), so inside
-// each cell run the slot is the rank of the original id (runs are short:
-// a cell holds ~8 particles at rest density).
+// Gather storage -> sorted order.  The sort leaves each cell's particles in
+// arrival (counting sort) or storage (radix sort) order; the reference sorts
+// (hash, original id), so inside each cell run the slot is the rank of the
+// original id (runs are short: a cell holds ~8 particles at rest density).
 __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ perm, const int2* __restrict__ range,
     const uint32_t* __restrict__ oid_a, const float4* __restrict__ pos_a, const float4* __restrict__ vel_a,
