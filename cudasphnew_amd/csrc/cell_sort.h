@@ -28,8 +28,11 @@ struct CellSort {
     // slots ordered by key; inside a cell in arrival order, which the
     // reorder pass replaces by its own tie order) and range[k] = [start,
     // end) for every occupied cell k.  range entries of other cells are not
-    // touched (the caller invalidates the previous step's cells).
-    void sort(hipStream_t st, const uint32_t* keys, uint32_t n, uint32_t* skey, uint32_t* perm, int2* range);
+    // touched, except that clear_prev = true first invalidates the cells
+    // of the previous sort of this key array (skey as it holds on entry,
+    // n keys); otherwise the caller does that.
+    void sort(hipStream_t st, const uint32_t* keys, uint32_t n, uint32_t* skey, uint32_t* perm, int2* range,
+              bool clear_prev);
     // The same with the count read on the device (n_dev <= capacity), so
     // that the call can be captured in a graph; grids sized for capacity.
     void sort(hipStream_t st, const uint32_t* keys, const uint32_t* n_dev, uint32_t capacity, uint32_t* skey,
@@ -37,7 +40,7 @@ struct CellSort {
 
   private:
     void launch(hipStream_t st, const uint32_t* keys, uint32_t n, const uint32_t* n_dev, uint32_t grid_n,
-                uint32_t* skey, uint32_t* perm, int2* range);
+                uint32_t* skey, uint32_t* perm, int2* range, const uint32_t* prev_skey, uint32_t prev_n);
 };
 
 }  // namespace sph

@@ -33,9 +33,17 @@ constexpr int kCsBlockCells = kCsThreads * kCsPerThread;   // 4096
 // in separate runs take separate atomics, which is still exact.
 __global__ __launch_bounds__(kCsThreads) void k_cs_count(const uint32_t* __restrict__ keys, uint32_t n_host,
                                                          const uint32_t* __restrict__ n_dev,
-                                                         uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank) {
+                                                         uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank,
+                                                         const uint32_t* __restrict__ prev_skey, uint32_t prev_n,
+                                                         int2* __restrict__ range) {
     const uint32_t n = n_dev ? *n_dev : n_host;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // invalidate the cells of the previous sort (its sorted keys, prev_n of
+    // them) before k_cs_ranges writes this sort's cells
+    if (prev_skey && i < prev_n) {
+        const uint32_t pk = prev_skey[i];
+        if (i == 0 || prev_skey[i - 1] != pk) range[pk] = make_int2(-1, -1);
+    }
     if ((i & ~63u) >= n) return;   // the whole wavefront lies past the end
     const int lane = threadIdx.x & 63;
     const bool valid = i < n;
@@ -184,20 +192,21 @@ void CellSort::release() {
 }
 
 void CellSort::sort(hipStream_t st, const uint32_t* keys, uint32_t n, uint32_t* skey, uint32_t* perm,
-                    int2* range) {
+                    int2* range, bool clear_prev) {
     if (n == 0) return;
-    launch(st, keys, n, nullptr, n, skey, perm, range);
+    launch(st, keys, n, nullptr, n, skey, perm, range, clear_prev ? skey : nullptr, n);
 }
 
 void CellSort::sort(hipStream_t st, const uint32_t* keys, const uint32_t* n_dev, uint32_t capacity, uint32_t* skey,
                     uint32_t* perm, int2* range) {
     if (capacity == 0) return;
-    launch(st, keys, 0, n_dev, capacity, skey, perm, range);
+    launch(st, keys, 0, n_dev, capacity, skey, perm, range, nullptr, 0);
 }
 
 void CellSort::launch(hipStream_t st, const uint32_t* keys, uint32_t n, const uint32_t* n_dev, uint32_t grid_n,
-                      uint32_t* skey, uint32_t* perm, int2* range) {
-    hipLaunchKernelGGL(k_cs_count, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, cnt, rank);
+                      uint32_t* skey, uint32_t* perm, int2* range, const uint32_t* prev_skey, uint32_t prev_n) {
+    hipLaunchKernelGGL(k_cs_count, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, cnt, rank,
+                       prev_skey, prev_n, range);
     hipLaunchKernelGGL(k_cs_reduce, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum);
     hipLaunchKernelGGL(k_cs_ranges, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum, range);
     hipLaunchKernelGGL(k_cs_scatter, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, rank, range,

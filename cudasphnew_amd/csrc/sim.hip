@@ -354,11 +354,12 @@ void Engine::enqueue_step(bool two_scale) {
     mark(-1);
     if (!keys_valid) launch_hash(stream, b);
     mark(kStageHash);
-    if (cells_valid) launch_cells_clear(stream, b);
+    // the counting sort clears the previous step's cells itself
+    if (cells_valid && !use_cell_sort) launch_cells_clear(stream, b);
     mark(kStageClear);
     if (use_cell_sort) {
         // the counting sort writes the cell ranges itself
-        csort.sort(stream, key, N, skey, perm, range);
+        csort.sort(stream, key, N, skey, perm, range, cells_valid);
         mark(kStageSort);
     } else {
         sorter.sort(stream, key, nullptr, skey, perm, n_dev, key_bits);
