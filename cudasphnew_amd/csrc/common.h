@@ -57,6 +57,9 @@ struct NeuralRefs {
     const float4* rec;      // per sorted slot: (gradient, value) of the learned distance
     uint8_t* need;          // per sorted slot: learned-source particle in the wall-penalty
                             // zone (written by the density pass, read by the network pass)
+    uint32_t* list_count;   // per scene: length of the network pass's particle list,
+                            // zeroed by the density pass that flags the particles
+    int nscenes;
 };
 
 __device__ __forceinline__ int wall_mode(const NeuralRefs& nr, uint32_t s) {

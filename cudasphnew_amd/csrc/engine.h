@@ -150,5 +150,6 @@ void neural_enqueue(Engine& e);
 size_t neural_bytes(const Engine& e);
 const float4* neural_records(const Engine& e);   // per sorted slot, or null
 uint8_t* neural_need(const Engine& e);           // per sorted slot, or null
+uint32_t* neural_list_count(const Engine& e);    // per scene, or null
 
 }  // namespace sph

@@ -36,7 +36,7 @@ struct StepBuffers {
     // read by the force pass; null = neighbour loops without masks
     uint64_t* masks = nullptr;
     // learned boundary records (mode == null: grid everywhere)
-    NeuralRefs nr = {nullptr, nullptr, nullptr, nullptr};
+    NeuralRefs nr = {nullptr, nullptr, nullptr, nullptr, nullptr, 0};
 };
 
 void launch_hash(hipStream_t st, const StepBuffers& b);

@@ -285,6 +285,10 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
 
 __device__ __forceinline__ float wall_density(const DevScene& S, const float4* __restrict__ grid4, float4 xi,
                                               const NeuralRefs& nr, uint32_t s) {
+    // the density pass flags the network's particles; it also starts their
+    // lists (the network pass appends after it)
+    if (s == 0 && nr.list_count)
+        for (int i = 0; i < nr.nscenes; ++i) nr.list_count[i] = 0u;
     const int wm = wall_mode(nr, s);
     if (wm == kWallNone) return 0.0f;
     float tx, ty, tz;

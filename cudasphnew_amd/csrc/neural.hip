@@ -630,11 +630,52 @@ __global__ __launch_bounds__(kThreads) void k_mlp_eval(const float4* __restrict_
 // learned-source cells that the density pass found in the wall-penalty
 // zone, one tile of 64 sorted particles per block; tiles without such a
 // particle exit at once.
+// Particles that take the network's normal this step (the density pass
+// flagged them in the penalty zone, their cell's source is the network),
+// packed per scene: scene s's entries at list[p_begin_s ..] with count
+// cnt[s] (zeroed by the density pass).  A wavefront whose slots lie in one
+// scene appends with one
+// atomic; the list order does not matter (each particle's row of the
+// network is computed independently of the other rows of its tile).
+__global__ __launch_bounds__(256) void k_wall_select(uint32_t n, const uint32_t* __restrict__ skey,
+                                                     const uint8_t* __restrict__ mode,
+                                                     const uint8_t* __restrict__ need,
+                                                     const DevScene* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ begins, int nscenes,
+                                                     const char* __restrict__ enabled, uint32_t* __restrict__ cnt,
+                                                     uint32_t* __restrict__ list) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((s & ~63u) >= n) return;   // the whole wavefront lies past the end
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    const int sc = valid ? scene_of(begins, nscenes, s) : -1;
+    const bool want = valid && enabled[sc] && need[s] && mode[skey[s]] == kWallNeural;
+    const uint64_t wants = __ballot(want);
+    if (!wants) return;
+    const int sc0 = __shfl(sc, 0, 64);
+    const uint64_t same = __ballot(!valid || sc == sc0);
+    if (same == ~0ull) {
+        // one scene: the first wanting lane reserves the run
+        const int leader = __ffsll((unsigned long long)wants) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[sc0], (uint32_t)__popcll(wants));
+        base = __shfl(base, leader, 64);
+        if (want) {
+            const uint64_t below = lane == 0 ? 0ull : (wants & (~0ull >> (64 - lane)));
+            list[scenes[sc0].p_begin + base + (uint32_t)__popcll(below)] = s;
+        }
+    } else if (want) {
+        list[scenes[sc].p_begin + atomicAdd(&cnt[sc], 1u)] = s;
+    }
+}
+
+// Network value and gradient for the listed particles: block b takes tile
+// (scene, p0) of the scene's slot range, i.e. list entries
+// [p0 - p_begin, p0 - p_begin + 64) of that scene; past the count it exits.
 __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__ tiles,
                                                        const float4* __restrict__ pos,
-                                                       const uint32_t* __restrict__ skey,
-                                                       const uint8_t* __restrict__ mode,
-                                                       const uint8_t* __restrict__ need,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint32_t* __restrict__ cnt,
                                                        const DevScene* __restrict__ scenes,
                                                        const NetView* __restrict__ nets,
                                                        float4* __restrict__ rec) {
@@ -642,24 +683,23 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint2 t = tiles[blockIdx.x];
     const DevScene& S = scenes[t.x];
-    const uint32_t p0 = t.y, end = S.p_end;
-    const uint32_t s = p0 + 16 * w + (lane & 15);
-    const bool valid = s < end;
-    const float4 x = pos[valid ? s : p0];
-    const bool mine = valid && need[s] && mode[skey[s]] == kWallNeural;
-    if (!__syncthreads_or(mine)) return;
+    const uint32_t k0 = t.y - S.p_begin, count = cnt[t.x];
+    if (k0 >= count) return;   // uniform over the block
+    const uint32_t* L = list + S.p_begin;
+    const uint32_t k = k0 + 16 * w + (lane & 15);
+    const float4 x = pos[L[k < count ? k : k0]];
     const NetView net = nets[t.x];
     float u[3], y[4], gu[4][3];
     normalized(net, x.x, x.y, x.z, u);
     mlp_value_grad(net, u, lane, w, sA1, sA2, sdZ, y, gu);
     if ((lane & 15) < 4) {
         const int j = lane & 15;
-        const uint32_t sj = p0 + 16 * w + 4 * (lane >> 4) + j;
-        if (sj < end && need[sj] && mode[skey[sj]] == kWallNeural) {
+        const uint32_t kj = k0 + 16 * w + 4 * (lane >> 4) + j;
+        if (kj < count) {
             float sc;
             const float d = decode(net, y[j], sc);
-            rec[sj] = make_float4(sc * gu[j][0] * 2.0f * net.inv_ext[0], sc * gu[j][1] * 2.0f * net.inv_ext[1],
-                                  sc * gu[j][2] * 2.0f * net.inv_ext[2], d);
+            rec[L[kj]] = make_float4(sc * gu[j][0] * 2.0f * net.inv_ext[0], sc * gu[j][1] * 2.0f * net.inv_ext[1],
+                                     sc * gu[j][2] * 2.0f * net.inv_ext[2], d);
         }
     }
 }
@@ -767,6 +807,9 @@ namespace sph {
 struct NeuralBoundary {
     float4* rec = nullptr;        // [N]
     uint8_t* need = nullptr;      // [N]
+    uint32_t* list = nullptr;     // [N] particles for the network, packed per scene
+    uint32_t* cnt = nullptr;      // [nscenes] list lengths
+    char* d_enabled = nullptr;    // [nscenes] scenes with a network
     nn::NetView* nets = nullptr;  // [nscenes]
     uint2* tiles = nullptr;
     uint32_t ntiles = 0;
@@ -779,7 +822,7 @@ void neural_destroy(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb) return;
     (void)hipStreamSynchronize(e.stream);
-    void* ptrs[] = {nb->rec, nb->need, nb->nets, nb->tiles};
+    void* ptrs[] = {nb->rec, nb->need, nb->list, nb->cnt, nb->d_enabled, nb->nets, nb->tiles};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     // the cells of the scenes that had a network go back to the grid
@@ -795,12 +838,15 @@ size_t neural_bytes(const Engine& e) { return e.nb ? e.nb->bytes : 0; }
 const float4* neural_records(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->rec : nullptr; }
 
 uint8_t* neural_need(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->need : nullptr; }
+uint32_t* neural_list_count(const Engine& e) { return e.nb && e.nb->ntiles ? e.nb->cnt : nullptr; }
 
 void neural_enqueue(Engine& e) {
     NeuralBoundary* nb = e.nb;
     if (!nb || !nb->ntiles) return;
-    hipLaunchKernelGGL(nn::k_wall_mlp, dim3(nb->ntiles), dim3(nn::kThreads), 0, e.stream, nb->tiles, e.pos_b, e.skey,
-                       e.wall_mode, nb->need, e.d_scenes, nb->nets, nb->rec);
+    hipLaunchKernelGGL(nn::k_wall_select, dim3((e.N + 255) / 256), dim3(256), 0, e.stream, e.N, e.skey, e.wall_mode,
+                       nb->need, e.d_scenes, e.d_begins, e.nscenes, nb->d_enabled, nb->cnt, nb->list);
+    hipLaunchKernelGGL(nn::k_wall_mlp, dim3(nb->ntiles), dim3(nn::kThreads), 0, e.stream, nb->tiles, e.pos_b,
+                       nb->list, nb->cnt, e.d_scenes, nb->nets, nb->rec);
 }
 
 static sph_status rebuild_tiles(Engine& e) {
@@ -835,13 +881,17 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         nb->enabled.assign(e.nscenes, 0);
         bool ok = hipMalloc(&nb->rec, sizeof(float4) * (size_t)e.N) == hipSuccess &&
                   hipMalloc(&nb->need, (size_t)e.N) == hipSuccess &&
+                  hipMalloc(&nb->list, sizeof(uint32_t) * (size_t)e.N) == hipSuccess &&
+                  hipMalloc(&nb->cnt, sizeof(uint32_t) * e.nscenes) == hipSuccess &&
+                  hipMalloc(&nb->d_enabled, e.nscenes) == hipSuccess &&
+                  hipMemset(nb->d_enabled, 0, e.nscenes) == hipSuccess &&
                   hipMalloc(&nb->nets, sizeof(nn::NetView) * e.nscenes) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
             neural_destroy(e);
             return e.fail(SPH_OUT_OF_MEMORY, "learned boundary buffers");
         }
-        nb->bytes = (sizeof(float4) + 1) * (size_t)e.N;
+        nb->bytes = (sizeof(float4) + 1 + sizeof(uint32_t)) * (size_t)e.N;
     }
     NeuralBoundary* nb = e.nb;
     const DevScene& S = e.hscenes[scene];
@@ -912,6 +962,9 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         if (p) (void)hipFree(p);
     if (st != SPH_OK) return st;
     nb->enabled[scene] = 1;
+    const char on = 1;
+    if (hipMemcpy(nb->d_enabled + scene, &on, 1, hipMemcpyHostToDevice) != hipSuccess)
+        return e.fail(SPH_DEVICE_ERROR, "network flag upload");
     st = rebuild_tiles(e);
     if (st != SPH_OK) return st;
     e.drop_graph();

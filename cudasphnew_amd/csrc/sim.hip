@@ -308,6 +308,8 @@ StepBuffers Engine::buffers() const {
     b.nr.skey = skey;
     b.nr.rec = neural_records(*this);
     b.nr.need = neural_need(*this);
+    b.nr.list_count = neural_list_count(*this);
+    b.nr.nscenes = nscenes;
     return b;
 }
 
