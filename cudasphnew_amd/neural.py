@@ -200,3 +200,33 @@ def train_for_scene(desc: SceneDesc, steps: int = 20000, batch: int = 65536, n_s
     net.set_samples(grid_samples(desc, n_samples, near_frac=0.3, near_width=3.0 * h, seed=seed + 1))
     net.train(steps, batch)
     return net
+
+
+def boundary_map_samples(bm: dict) -> np.ndarray:
+    """(n, 4) float32 (x, y, z, distance) at the nodes of a generated boundary
+    map (cudasphnew_amd.meshsdf.generate) that hold a distance (present), the
+    band within max(restDistance, compactSupport) of the mesh."""
+    dims = bm["dims"]
+    lo = np.asarray(bm["lo"], np.float32)
+    hi = np.asarray(bm["hi"], np.float32)
+    iz, iy, ix = np.nonzero(bm["present"])
+    sp = (hi - lo) / (np.asarray(dims, np.float32) - 1)
+    out = np.empty((ix.shape[0], 4), np.float32)
+    out[:, 0] = lo[0] + ix * sp[0]
+    out[:, 1] = lo[1] + iy * sp[1]
+    out[:, 2] = lo[2] + iz * sp[2]
+    out[:, 3] = bm["distance"][iz, iy, ix]
+    return out
+
+
+def train_for_boundary_map(bm: dict, band: float, steps: int = 20000, batch: int = 65536, seed: int = 1,
+                           samples: np.ndarray | None = None) -> NeuralSDF:
+    """The learned SDF of a mesh: trained on the band nodes of its generated
+    boundary map (or on given samples of them), over the map's box, with the
+    band-compressed target tanh(d / band); Adam lr 1e-3 with cosine decay to
+    1e-5 (the recipe of train_for_scene)."""
+    net = NeuralSDF(bm["lo"], bm["hi"], seed=seed, out_scale=band, lr=1e-3, lr_min=1e-5, decay_steps=steps,
+                    band=band)
+    net.set_samples(boundary_map_samples(bm) if samples is None else samples)
+    net.train(steps, batch)
+    return net

@@ -72,3 +72,34 @@ def test_generated_map_save_load(tmp_path):
     assert np.array_equal(bm2["present"], bm["present"])
     assert np.allclose(bm2["distance"], bm["distance"], rtol=1e-7, atol=0)
     assert np.allclose(bm2["normal"], bm["normal"], rtol=1e-7, atol=0)
+
+
+def test_network_fits_a_generated_mesh_sdf():
+    """The learned SDF trained on the band nodes of a generated boundary map
+    (the reference's example configuration, icosphere, 266^3 nodes): RMS
+    error over held-out band nodes within 0.05 compactSupport, and the sign
+    right at 98% of the held-out nodes within compactSupport of the surface
+    (the ones it misses lie within the network's largest error, ~0.3 cs, of
+    the surface)."""
+    from cudasphnew_amd import neural as N
+    v = M.fit(MESHES["icosphere_vertices"], (-0.75, -0.75, -0.75), (0.75, 0.75, 0.75))
+    f = MESHES["icosphere_faces"]
+    cs = 0.017
+    bm = M.generate(v, f, M.config(2 * cs, cs, cs / 3))
+    smp = N.boundary_map_samples(bm)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(smp.shape[0])
+    test, train = smp[perm[: smp.shape[0] // 10]], smp[perm[smp.shape[0] // 10:]]
+    net = N.train_for_boundary_map(bm, band=2 * cs, samples=train)
+    pred = net.eval(test[:, :3])[:, 0]
+    err = np.abs(pred - test[:, 3])
+    near = np.abs(test[:, 3]) < cs
+    rms = float(np.sqrt((err ** 2).mean()))
+    sign = float((np.sign(pred[near]) == np.sign(test[near, 3])).mean())
+    print(f"icosphere band nodes {smp.shape[0]}: rms {rms / cs:.4f} cs, p99 {np.quantile(err, 0.99) / cs:.3f} cs, "
+          f"sign agreement {sign:.4f}")
+    assert rms <= 0.05 * cs
+    assert sign >= 0.98
+    wrong = near.copy()
+    wrong[near] = np.sign(pred[near]) != np.sign(test[near, 3])
+    assert np.all(np.abs(test[wrong, 3]) <= 0.35 * cs)
