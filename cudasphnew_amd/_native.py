@@ -188,6 +188,12 @@ class SphBoundaryMapConfig(C.Structure):
     _fields_ = [("rest_distance", C.c_float), ("compact_support", C.c_float), ("dx", C.c_float)]
 
 
+class SphMapSolid(C.Structure):
+    _fields_ = [("dims", i3), ("lo", f3), ("dx", C.c_float), ("band", C.c_float),
+                ("distance", C.POINTER(C.c_float)), ("present", C.POINTER(C.c_uint8)),
+                ("solid_lo", f3), ("solid_hi", f3)]
+
+
 class SphError(RuntimeError):
     """Raised for a non-OK sph_status (the reference throws
     std::runtime_error from CUDA_SAFE_CALL, util.h:24-39)."""
@@ -256,6 +262,8 @@ SIGNATURES = [
     ("sph_scene_particles", _st, [C.POINTER(SceneDesc), fptr, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("sph_scene_boundary", _st, [C.POINTER(SceneDesc), fptr, fptr, fptr]),
     ("sph_scene_sdf", _st, [C.POINTER(SceneDesc), fptr, C.c_size_t, fptr]),
+    ("sph_scene_boundary_maps", _st, [C.POINTER(SceneDesc), C.POINTER(SphMapSolid), C.c_int32, fptr, fptr, fptr]),
+    ("sph_scene_sdf_maps", _st, [C.POINTER(SceneDesc), C.POINTER(SphMapSolid), C.c_int32, fptr, C.c_size_t, fptr]),
     ("sph_example01_boundary", _st, [i3, f3, fptr, fptr, fptr, fptr, fptr]),
     ("sph_scene_grid_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
     ("sph_scene_sdf_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
@@ -281,6 +289,8 @@ SIGNATURES = [
     ("sph_boundary_map_domain", _st, [C.POINTER(SphMesh), C.c_float, i3, f3, f3]),
     ("sph_boundary_map_generate", _st, [C.POINTER(SphMesh), C.POINTER(SphBoundaryMapConfig), fptr, fptr,
                                         C.POINTER(C.c_uint8), fptr]),
+    ("sph_boundary_map_generate_on", _st, [C.POINTER(SphMesh), C.POINTER(SphBoundaryMapConfig), i3, f3, fptr,
+                                           fptr, C.POINTER(C.c_uint8), fptr]),
     ("sph_boundary_map_save", _st, [C.c_char_p, C.POINTER(SphBoundaryMapConfig), i3, f3, f3, fptr, fptr,
                                     C.POINTER(C.c_uint8)]),
     ("sph_boundary_map_load", _st, [C.c_char_p, C.POINTER(SphBoundaryMapConfig), i3, f3, f3, fptr, fptr,

@@ -88,18 +88,22 @@ struct Domain {
 };
 
 // Node box of a triangle (computeGridCoordinates, boundary_map.cpp:498-536):
-// bounds grown by maxDist + dx, clamped to the domain, truncated.
-inline void triangle_box(const Domain& D, const float hi[3], const float* t1, const float* t2, const float* t3,
+// bounds grown by maxDist + dx, clamped to the domain, truncated.  False
+// when the grown bounds miss the domain (possible on a caller-given grid),
+// so that no node lies within maxDist of the triangle.
+inline bool triangle_box(const Domain& D, const float hi[3], const float* t1, const float* t2, const float* t3,
                          unsigned cmin[3], unsigned cmax[3]) {
     const float delta = D.maxdist + D.dx;
     for (int a = 0; a < 3; ++a) {
         float mn = std::min(t1[a], std::min(t2[a], t3[a])) - delta;
         float mx = std::max(t1[a], std::max(t2[a], t3[a])) + delta;
+        if (mx < D.lo[a] || mn > hi[a]) return false;
         mn = std::max(mn, D.lo[a]);
         mx = std::min(mx, hi[a]);
         cmin[a] = std::min((unsigned)((mn - D.lo[a]) / D.dx), (unsigned)D.dims[a] - 1u);
         cmax[a] = std::min((unsigned)((mx - D.lo[a]) / D.dx), (unsigned)D.dims[a] - 1u);
     }
+    return true;
 }
 
 // One work item: a run of up to kChunk nodes of one triangle's node box.
@@ -321,19 +325,19 @@ sph_status sph_boundary_map_domain(const sph_mesh* m, float dx, int32_t dims[3],
     return SPH_OK;
 }
 
-sph_status sph_boundary_map_generate(const sph_mesh* m, const sph_boundary_map_config* cfg, float* distance,
-                                     float* normal, uint8_t* present, float* ms) {
-    if (!mesh_ok(m) || !cfg || !distance || !normal || !present) return SPH_INVALID_ARG;
-    if (!(cfg->dx > 0.0f) || !(cfg->compact_support > 0.0f) || !(cfg->rest_distance >= 0.0f)) return SPH_INVALID_ARG;
+static bool config_ok(const sph_boundary_map_config* cfg) {
+    return cfg && cfg->dx > 0.0f && cfg->compact_support > 0.0f && cfg->rest_distance >= 0.0f &&
+           std::isfinite(cfg->dx) && std::isfinite(cfg->compact_support) && std::isfinite(cfg->rest_distance);
+}
+
+// Both generators: the field over domain D (D.lo .. hi, D.dims nodes).
+static sph_status generate_on(const sph_mesh* m, const sph_boundary_map_config* cfg, Domain D, const float hi[3],
+                              float* distance, float* normal, uint8_t* present, float* ms) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
         (void)hipGetLastError();
         return SPH_NO_DEVICE;
     }
-    Domain D;
-    float hi[3];
-    sph_status st = sph_boundary_map_domain(m, cfg->dx, D.dims, D.lo, hi);
-    if (st != SPH_OK) return st;
     D.dx = cfg->dx;
     D.maxdist = std::max(cfg->rest_distance, cfg->compact_support);
     const size_t nodes = (size_t)D.dims[0] * D.dims[1] * D.dims[2];
@@ -349,7 +353,7 @@ sph_status sph_boundary_map_generate(const sph_mesh* m, const sph_boundary_map_c
         tris[f].b = make_float3(p2[0], p2[1], p2[2]);
         tris[f].c = make_float3(p3[0], p3[1], p3[2]);
         unsigned cmin[3], cmax[3];
-        triangle_box(D, hi, p1, p2, p3, cmin, cmax);
+        if (!triangle_box(D, hi, p1, p2, p3, cmin, cmax)) continue;
         Item it;
         it.tri = f;
         uint32_t total = 1;
@@ -442,6 +446,31 @@ sph_status sph_boundary_map_generate(const sph_mesh* m, const sph_boundary_map_c
         return SPH_DEVICE_ERROR;
     }
     return SPH_OK;
+}
+
+sph_status sph_boundary_map_generate(const sph_mesh* m, const sph_boundary_map_config* cfg, float* distance,
+                                     float* normal, uint8_t* present, float* ms) {
+    if (!mesh_ok(m) || !config_ok(cfg) || !distance || !normal || !present) return SPH_INVALID_ARG;
+    Domain D;
+    float hi[3];
+    sph_status st = sph_boundary_map_domain(m, cfg->dx, D.dims, D.lo, hi);
+    if (st != SPH_OK) return st;
+    return generate_on(m, cfg, D, hi, distance, normal, present, ms);
+}
+
+sph_status sph_boundary_map_generate_on(const sph_mesh* m, const sph_boundary_map_config* cfg,
+                                        const int32_t dims[3], const float lo[3], float* distance, float* normal,
+                                        uint8_t* present, float* ms) {
+    if (!mesh_ok(m) || !config_ok(cfg) || !dims || !lo || !distance || !normal || !present) return SPH_INVALID_ARG;
+    Domain D;
+    float hi[3];
+    for (int a = 0; a < 3; ++a) {
+        if (dims[a] < 1 || !std::isfinite(lo[a])) return SPH_INVALID_ARG;
+        D.dims[a] = dims[a];
+        D.lo[a] = lo[a];
+        hi[a] = lo[a] + (float)(dims[a] - 1) * cfg->dx;
+    }
+    return generate_on(m, cfg, D, hi, distance, normal, present, ms);
 }
 
 // BoundaryMap::Save text format (boundary_map.cpp:161-221): restDistance,

@@ -267,16 +267,15 @@ void container_ghosts(const float c[3], const float e[3], float support, float s
             }
 }
 
-// Ghost lattice of an obstacle: lattice over the solid's bounding box grown
-// by support + spacing; a point is a ghost if it lies in the solid within
-// that depth and inside the container (the container's own ghosts cover the
-// region outside it).
-template <class Emit>
-void obstacle_ghosts(const sph_scene_desc& dsc, const sph_solid& s, float support, float spacing,
-                     Emit&& emit) {
+// Ghost lattice of a solid: lattice over the solid's bounding box [blo,
+// bhi] grown by support + spacing; a point is a ghost if it lies in the
+// solid (sdf > 0 inside) within that depth and inside the container (the
+// container's own ghosts cover the region outside it).
+template <class Sdf, class Emit>
+void solid_ghosts(const sph_scene_desc& dsc, const float blo[3], const float bhi[3], Sdf&& sdf, float support,
+                  float spacing, Emit&& emit) {
     float d = support + spacing;
     float lo[3], hi[3];
-    solid_bbox(s, lo, hi);
     float c[3], e[3];
     for (int i = 0; i < 3; ++i) {
         c[i] = 0.5f * (dsc.box_lo[i] + dsc.box_hi[i]);
@@ -284,8 +283,8 @@ void obstacle_ghosts(const sph_scene_desc& dsc, const sph_solid& s, float suppor
     }
     unsigned n[3];
     for (int a = 0; a < 3; ++a) {
-        lo[a] -= d;
-        hi[a] += d;
+        lo[a] = blo[a] - d;
+        hi[a] = bhi[a] + d;
         n[a] = (unsigned)((hi[a] - lo[a]) / spacing);
     }
     for (unsigned k = 0; k <= n[2]; ++k)
@@ -293,9 +292,121 @@ void obstacle_ghosts(const sph_scene_desc& dsc, const sph_solid& s, float suppor
             for (unsigned i = 0; i <= n[0]; ++i) {
                 float p[3] = {lo[0] + (float)i * spacing, lo[1] + (float)j * spacing,
                               lo[2] + (float)k * spacing};
-                float v = solid_sdf(s, p);
+                float v = sdf(p);
                 if (v >= 0.0f && v <= d && inside_box_open(p, c, e)) emit(p);
             }
+}
+
+template <class Emit>
+void obstacle_ghosts(const sph_scene_desc& dsc, const sph_solid& s, float support, float spacing,
+                     Emit&& emit) {
+    float lo[3], hi[3];
+    solid_bbox(s, lo, hi);
+    solid_ghosts(dsc, lo, hi, [&](const float p[3]) { return solid_sdf(s, p); }, support, spacing, emit);
+}
+
+// Particle mass: explicit, else rho0 times the lattice cell volume.
+float particle_mass_of(const sph_scene_desc& d) {
+    return d.particle_mass > 0.0f ? d.particle_mass
+                                  : d.rest_density * d.lattice_spacing * d.lattice_spacing * d.lattice_spacing;
+}
+
+// Ghost lattice spacing: explicit, else std::powf(m/rho0, 1/3) (cu:2598).
+float ghost_spacing_of(const sph_scene_desc& d) {
+    return d.ghost_spacing > 0.0f ? d.ghost_spacing : std::pow(particle_mass_of(d) / d.rest_density, 1.0f / 3.0f);
+}
+
+// A solid given by a sampled signed distance (sph_map_solid): negative
+// inside at the nodes, trilinear inside a cell whose eight nodes all hold a
+// distance.  Elsewhere the point is at least `band` from the surface and
+// takes +-band with the side of its nearest node.  Nodes without a distance
+// take the side of the node they are first reached from in a breadth-first
+// walk over the six neighbours that starts from every node with one: the
+// band of such nodes separates the inside from the outside, so each
+// connected region of the others lies on one side.  (With no node holding
+// a distance everything is outside.)  Points off the grid are outside.
+struct MapSolid {
+    const sph_map_solid* m = nullptr;
+    std::vector<uint8_t> inside;   // per node
+
+    size_t at(int i, int j, int k) const { return i + (size_t)m->dims[0] * (j + (size_t)m->dims[1] * k); }
+    bool present(size_t q) const { return !m->present || m->present[q]; }
+
+    void prepare(const sph_map_solid* map) {
+        m = map;
+        const size_t n = (size_t)m->dims[0] * m->dims[1] * m->dims[2];
+        inside.assign(n, 0);
+        std::vector<uint8_t> seen(n, 0);
+        std::vector<size_t> queue;
+        queue.reserve(n);
+        for (size_t q = 0; q < n; ++q)
+            if (present(q)) {
+                inside[q] = m->distance[q] < 0.0f;
+                seen[q] = 1;
+                queue.push_back(q);
+            }
+        const int nx = m->dims[0], ny = m->dims[1], nz = m->dims[2];
+        for (size_t h = 0; h < queue.size(); ++h) {
+            const size_t q = queue[h];
+            const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+            const int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+            for (const auto& c : nb) {
+                if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= nx || c[1] >= ny || c[2] >= nz) continue;
+                const size_t r = at(c[0], c[1], c[2]);
+                if (seen[r]) continue;
+                seen[r] = 1;
+                inside[r] = inside[q];
+                queue.push_back(r);
+            }
+        }
+    }
+
+    // > 0 inside the solid (scene convention).
+    float sdf(const float p[3]) const {
+        int i0[3];
+        float w[3];
+        int nearest[3];
+        for (int a = 0; a < 3; ++a) {
+            const float t = (p[a] - m->lo[a]) / m->dx;
+            if (!(t >= 0.0f && t <= (float)(m->dims[a] - 1))) return -m->band;
+            i0[a] = std::min((int)t, m->dims[a] - 2);
+            w[a] = t - (float)i0[a];
+            nearest[a] = std::min((int)(t + 0.5f), m->dims[a] - 1);
+        }
+        float c[8];
+        for (int v = 0; v < 8; ++v) {
+            const size_t q = at(i0[0] + (v & 1), i0[1] + ((v >> 1) & 1), i0[2] + (v >> 2));
+            if (!present(q)) {
+                const bool in = inside[at(nearest[0], nearest[1], nearest[2])];
+                return in ? m->band : -m->band;
+            }
+            c[v] = m->distance[q];
+        }
+        auto lerp = [](float a, float b, float t) { return a + t * (b - a); };
+        const float x00 = lerp(c[0], c[1], w[0]), x10 = lerp(c[2], c[3], w[0]);
+        const float x01 = lerp(c[4], c[5], w[0]), x11 = lerp(c[6], c[7], w[0]);
+        return -lerp(lerp(x00, x10, w[1]), lerp(x01, x11, w[1]), w[2]);
+    }
+};
+
+// A map solid is usable by a scene when every point the grid builder and
+// the ghost lattice look at (within support + ghost spacing of the solid)
+// lies on the grid in a cell of nodes that hold distances: the grid covers
+// the solid's box grown by that depth and the band exceeds it by a cell
+// diagonal.
+bool map_ok(const sph_map_solid& m, float support, float ghost_spacing) {
+    if (!m.distance || !(m.dx > 0.0f) || !std::isfinite(m.dx) || !(m.band > 0.0f) || !std::isfinite(m.band))
+        return false;
+    const float depth = support + ghost_spacing;
+    if (!(m.band >= depth + std::sqrt(3.0f) * m.dx)) return false;
+    for (int a = 0; a < 3; ++a) {
+        if (m.dims[a] < 2 || !std::isfinite(m.lo[a]) || !std::isfinite(m.solid_lo[a]) ||
+            !std::isfinite(m.solid_hi[a]) || !(m.solid_lo[a] <= m.solid_hi[a]))
+            return false;
+        const float hi = m.lo[a] + (float)(m.dims[a] - 1) * m.dx;
+        if (!(m.lo[a] <= m.solid_lo[a] - depth && hi >= m.solid_hi[a] + depth)) return false;
+    }
+    return (size_t)m.dims[0] * m.dims[1] * m.dims[2] <= ((size_t)1 << 31);
 }
 
 bool finite3(const float* v) {
@@ -652,11 +763,10 @@ sph_status sph_scene_grid_samples(const sph_scene_desc* d, uint64_t seed, size_t
     return SPH_OK;
 }
 
-sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* density, float* viscosity) {
-    if (!d) return SPH_INVALID_ARG;
-    for (int a = 0; a < 3; ++a)
-        if (d->boundary_dims[a] < 2) return SPH_INVALID_ARG;
-    if (!(d->boundary_spacing > 0.0f) || !finite3(d->boundary_origin)) return SPH_INVALID_ARG;
+// The three grids of a scene whose solids are the container, the analytic
+// obstacles and `maps` (prepared map solids).
+static sph_status build_boundary(const sph_scene_desc* d, const std::vector<MapSolid>& maps, float* distance,
+                                 float* density, float* viscosity) {
     GridBuilder g;
     for (int a = 0; a < 3; ++a) {
         g.start[a] = d->boundary_origin[a];
@@ -666,11 +776,13 @@ sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* d
     g.rest_distance = d->rest_distance;
     g.support = d->boundary_support;
     g.reset();
-    g.set_distances([&](const float p[3]) { return scene_sdf_point(*d, p); });
-    float mass = d->particle_mass > 0.0f
-                     ? d->particle_mass
-                     : d->rest_density * d->lattice_spacing * d->lattice_spacing * d->lattice_spacing;
-    float gsp = d->ghost_spacing > 0.0f ? d->ghost_spacing : std::pow(mass / d->rest_density, 1.0f / 3.0f);
+    g.set_distances([&](const float p[3]) {
+        float v = scene_sdf_point(*d, p);
+        for (const MapSolid& m : maps) v = std::max(v, m.sdf(p));
+        return v;
+    });
+    const float mass = particle_mass_of(*d);
+    const float gsp = ghost_spacing_of(*d);
     float c[3], e[3];
     for (int a = 0; a < 3; ++a) {
         c[a] = 0.5f * (d->box_lo[a] + d->box_hi[a]);
@@ -680,8 +792,57 @@ sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* d
     container_ghosts(c, e, g.support, gsp, emit);
     for (int k = 0; k < d->num_obstacles && k < SPH_MAX_OBSTACLES; ++k)
         obstacle_ghosts(*d, d->obstacles[k], g.support, gsp, emit);
+    for (const MapSolid& m : maps)
+        solid_ghosts(*d, m.m->solid_lo, m.m->solid_hi, [&](const float p[3]) { return m.sdf(p); }, g.support, gsp,
+                     emit);
     g.finish(mass, d->viscosity, d->rest_density);
     g.flatten(distance, density, viscosity);
+    return SPH_OK;
+}
+
+static bool boundary_desc_ok(const sph_scene_desc* d) {
+    if (!d) return false;
+    for (int a = 0; a < 3; ++a)
+        if (d->boundary_dims[a] < 2) return false;
+    return d->boundary_spacing > 0.0f && finite3(d->boundary_origin);
+}
+
+static sph_status prepare_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                               std::vector<MapSolid>& out) {
+    if (num_maps < 0 || (num_maps > 0 && !maps)) return SPH_INVALID_ARG;
+    out.resize((size_t)num_maps);
+    for (int32_t k = 0; k < num_maps; ++k) {
+        if (!map_ok(maps[k], d->boundary_support, ghost_spacing_of(*d))) return SPH_INVALID_ARG;
+        out[k].prepare(&maps[k]);
+    }
+    return SPH_OK;
+}
+
+sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* density, float* viscosity) {
+    if (!boundary_desc_ok(d)) return SPH_INVALID_ARG;
+    return build_boundary(d, {}, distance, density, viscosity);
+}
+
+sph_status sph_scene_boundary_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                                   float* distance, float* density, float* viscosity) {
+    if (!boundary_desc_ok(d)) return SPH_INVALID_ARG;
+    std::vector<MapSolid> ms;
+    sph_status st = prepare_maps(d, maps, num_maps, ms);
+    if (st != SPH_OK) return st;
+    return build_boundary(d, ms, distance, density, viscosity);
+}
+
+sph_status sph_scene_sdf_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                              const float* xyz, size_t n, float* out) {
+    if (!d || (n && (!xyz || !out))) return SPH_INVALID_ARG;
+    std::vector<MapSolid> ms;
+    sph_status st = prepare_maps(d, maps, num_maps, ms);
+    if (st != SPH_OK) return st;
+    for (size_t i = 0; i < n; ++i) {
+        float v = scene_sdf_point(*d, xyz + 3 * i);
+        for (const MapSolid& m : ms) v = std::max(v, m.sdf(xyz + 3 * i));
+        out[i] = v;
+    }
     return SPH_OK;
 }
 

@@ -65,6 +65,37 @@ def generate(vertices, faces, cfg: SphBoundaryMapConfig):
     return {"dims": dims, "lo": lo, "hi": hi, "distance": d, "normal": n, "present": p, "ms": float(ms.value)}
 
 
+def grown_domain(vertices, dx: float, margin: float):
+    """(dims, lo) of a grid of spacing dx over the mesh's bounding box grown
+    by margin on every side."""
+    v = np.ascontiguousarray(vertices, np.float32)
+    vlo, vhi = v.min(0), v.max(0)
+    lo = (vlo - np.float32(margin)).astype(np.float32)
+    dims = tuple(int(np.ceil((float(vhi[a]) - float(vlo[a]) + 2 * margin) / dx)) + 1 for a in range(3))
+    return dims, lo
+
+
+def generate_on(vertices, faces, cfg: SphBoundaryMapConfig, dims, lo):
+    """As generate, on a caller-given grid (dims nodes from lo, spacing
+    cfg.dx); the dict also holds the mesh's bounding box (mesh_lo,
+    mesh_hi)."""
+    m, keep = _mesh(vertices, faces)
+    dims = tuple(int(x) for x in dims)
+    lo = np.asarray(lo, np.float32)
+    shape = (dims[2], dims[1], dims[0])
+    d = np.empty(shape, np.float32)
+    n = np.empty(shape + (3,), np.float32)
+    p = np.empty(shape, np.uint8)
+    ms = C.c_float()
+    _check(lib.sph_boundary_map_generate_on(C.byref(m), C.byref(cfg), (C.c_int32 * 3)(*dims),
+                                            (C.c_float * 3)(*(float(x) for x in lo)), as_fptr(d), as_fptr(n),
+                                            p.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(ms)), "generate_on")
+    v = keep[0]
+    hi = (lo + np.float32(cfg.dx) * (np.array(dims, np.float32) - 1)).astype(np.float32)
+    return {"dims": dims, "lo": lo, "hi": hi, "distance": d, "normal": n, "present": p, "ms": float(ms.value),
+            "mesh_lo": v.min(0), "mesh_hi": v.max(0)}
+
+
 def save(path: str, cfg, bm: dict) -> None:
     _check(lib.sph_boundary_map_save(path.encode(), C.byref(cfg), (C.c_int32 * 3)(*bm["dims"]),
                                      (C.c_float * 3)(*bm["lo"]), (C.c_float * 3)(*bm["hi"]),

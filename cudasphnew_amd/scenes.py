@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._native import (Config, SceneDesc, SphBoundary, SphParams, SphScene, as_fptr, check, lib)
+from ._native import (Config, SceneDesc, SphBoundary, SphMapSolid, SphParams, SphScene, as_fptr, check, lib)
 
 
 @dataclass
@@ -91,17 +91,86 @@ def particles_of(desc: SceneDesc) -> np.ndarray:
     return pos
 
 
-def boundary_of(desc: SceneDesc):
+def ghost_spacing_of(desc: SceneDesc) -> float:
+    """Spacing of the grid builder's ghost lattice: explicit, else
+    (m/rho0)^(1/3)."""
+    if desc.ghost_spacing > 0:
+        return float(desc.ghost_spacing)
+    m = desc.particle_mass if desc.particle_mass > 0 else desc.rest_density * desc.lattice_spacing ** 3
+    return float((m / desc.rest_density) ** (1.0 / 3.0))
+
+
+def map_solid(bm: dict, band: float, solid_lo=None, solid_hi=None):
+    """sph_map_solid over a boundary map dict (meshsdf.generate_on /
+    generate / load): the solid's box defaults to the mesh's (mesh_lo,
+    mesh_hi) or the map's domain.  Returns (struct, arrays to keep alive)."""
+    d = np.ascontiguousarray(bm["distance"], np.float32)
+    p = np.ascontiguousarray(bm["present"], np.uint8)
+    m = SphMapSolid()
+    for a in range(3):
+        m.dims[a] = int(bm["dims"][a])
+        m.lo[a] = float(bm["lo"][a])
+    m.dx = float(bm["dx"]) if "dx" in bm else float((bm["hi"][0] - bm["lo"][0]) / (bm["dims"][0] - 1))
+    m.band = float(band)
+    m.distance = as_fptr(d)
+    m.present = p.ctypes.data_as(C.POINTER(C.c_uint8))
+    if solid_lo is None:
+        solid_lo = bm["mesh_lo"] if "mesh_lo" in bm else bm["lo"]
+    if solid_hi is None:
+        solid_hi = bm["mesh_hi"] if "mesh_hi" in bm else bm["hi"]
+    for a in range(3):
+        m.solid_lo[a] = float(solid_lo[a])
+        m.solid_hi[a] = float(solid_hi[a])
+    return m, (d, p)
+
+
+def mesh_obstacle(vertices, faces, desc: SceneDesc, dx: float | None = None):
+    """Boundary map of a mesh (already placed, e.g. by meshsdf.fit) made to
+    serve as an obstacle of the scene `desc`: generated on the GPU on a grid
+    of spacing dx (default half the scene's boundary spacing) over the
+    mesh's box grown by the band, band = support + ghost spacing + 2 dx.
+    Returns the map dict with 'dx' and 'band' added."""
+    from . import meshsdf
+    if dx is None:
+        dx = float(desc.boundary_spacing) / 2
+    band = float(desc.boundary_support) + ghost_spacing_of(desc) + 2.0 * dx
+    cfg = meshsdf.config(float(desc.rest_distance), band, dx)
+    dims, lo = meshsdf.grown_domain(vertices, dx, band + dx)
+    bm = meshsdf.generate_on(vertices, faces, cfg, dims, lo)
+    bm["dx"] = float(cfg.dx)
+    bm["band"] = float(np.float32(band))
+    return bm
+
+
+def _maps(desc, maps):
+    maps = list(maps or [])
+    structs, keep = [], []
+    for bm in maps:
+        m, k = map_solid(bm, bm["band"]) if isinstance(bm, dict) else (bm, ())
+        structs.append(m)
+        keep.append(k)
+    arr = (SphMapSolid * max(1, len(structs)))(*structs)
+    return arr, len(structs), keep
+
+
+def boundary_of(desc: SceneDesc, maps=None):
+    """Distance, density and viscosity grids (nz, ny, nx) of the scene;
+    `maps`: extra obstacles given as boundary map dicts (mesh_obstacle) or
+    SphMapSolid structs."""
     nx, ny, nz = (int(v) for v in desc.boundary_dims)
     out = [np.empty((nz, ny, nx), np.float32) for _ in range(3)]
-    check(lib.sph_scene_boundary(C.byref(desc), *(as_fptr(a) for a in out)))
+    if not maps:
+        check(lib.sph_scene_boundary(C.byref(desc), *(as_fptr(a) for a in out)))
+    else:
+        arr, n, keep = _maps(desc, maps)
+        check(lib.sph_scene_boundary_maps(C.byref(desc), arr, n, *(as_fptr(a) for a in out)))
     return out
 
 
-def scene_from_desc(desc: SceneDesc) -> Scene:
+def scene_from_desc(desc: SceneDesc, maps=None) -> Scene:
     params = params_of(desc)
     pos = particles_of(desc)
-    d, w, v = boundary_of(desc)
+    d, w, v = boundary_of(desc, maps)
     return Scene(desc, params, pos, d, w, v)
 
 
@@ -112,10 +181,14 @@ def make_scene(config, index: int = 0, seed: int = 1) -> Scene:
     return scene_from_desc(config_desc(config, index, seed))
 
 
-def scene_sdf(desc: SceneDesc, xyz: np.ndarray) -> np.ndarray:
+def scene_sdf(desc: SceneDesc, xyz: np.ndarray, maps=None) -> np.ndarray:
     pts = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
     out = np.empty(pts.shape[0], np.float32)
-    check(lib.sph_scene_sdf(C.byref(desc), as_fptr(pts), pts.shape[0], as_fptr(out)))
+    if not maps:
+        check(lib.sph_scene_sdf(C.byref(desc), as_fptr(pts), pts.shape[0], as_fptr(out)))
+    else:
+        arr, n, keep = _maps(desc, maps)
+        check(lib.sph_scene_sdf_maps(C.byref(desc), arr, n, as_fptr(pts), pts.shape[0], as_fptr(out)))
     return out
 
 

@@ -447,6 +447,39 @@ sph_status sph_scene_boundary(const sph_scene_desc* desc, float* distance,
 sph_status sph_scene_sdf(const sph_scene_desc* desc, const float* xyz,
                          size_t n, float* out);
 
+/* An obstacle given by a sampled signed distance, e.g. a mesh's boundary
+ * map (sph_boundary_map_generate_on).  No reference counterpart: the
+ * reference generates mesh maps (main.cpp:91-105) but its grid builder
+ * only knows boxes.  Nodes at lo + dx (i, j, k), x fastest; distance
+ * negative inside.  Inside a cell whose eight nodes hold distances the
+ * solid's distance is trilinear; elsewhere a point is taken to be at least
+ * `band` from the surface, on the side of its nearest node (nodes without
+ * a distance take the side of the band region they border), and points off
+ * the grid are outside.  Usable when band >= support + ghost spacing +
+ * sqrt(3) dx and the grid covers [solid_lo, solid_hi] grown by support +
+ * ghost spacing (else SPH_INVALID_ARG). */
+typedef struct sph_map_solid {
+    int32_t dims[3];
+    float lo[3];
+    float dx;
+    float band;                 /* distances are exact within band (maxDist) */
+    const float* distance;      /* dims[0]*dims[1]*dims[2] */
+    const uint8_t* present;     /* 1 = node holds a distance; NULL = all do */
+    float solid_lo[3];          /* the solid's bounding box: the ghost      */
+    float solid_hi[3];          /* lattice starts at solid_lo - (cs + gsp)  */
+} sph_map_solid;
+/* sph_scene_boundary with map solids added to the desc's obstacles (their
+ * ghosts after the analytic obstacles').  num_maps = 0 gives
+ * sph_scene_boundary's grids bit for bit. */
+sph_status sph_scene_boundary_maps(const sph_scene_desc* desc,
+                                   const sph_map_solid* maps, int32_t num_maps,
+                                   float* distance, float* density,
+                                   float* viscosity);
+/* sph_scene_sdf with the map solids in the union. */
+sph_status sph_scene_sdf_maps(const sph_scene_desc* desc,
+                              const sph_map_solid* maps, int32_t num_maps,
+                              const float* xyz, size_t n, float* out);
+
 /* Example01's own grid builder call (cu:2592-2612): box of half extents
  * (0.7,0.5,0.3) on the grid (-0.8,-0.6,-0.4)..(0.8,0.6,0.4), spacing h/4.
  * Writes dims and, if the pointers are non-NULL, the node data. */
@@ -619,6 +652,18 @@ sph_status sph_boundary_map_generate(const sph_mesh* mesh,
                                      const sph_boundary_map_config* cfg,
                                      float* distance, float* normal,
                                      uint8_t* present, float* ms);
+/* The same field on a caller-given grid: dims[0]*dims[1]*dims[2] nodes at
+ * lo + dx (i, j, k), dx = cfg->dx.  No reference counterpart (AddCanvas
+ * always takes the mesh's bounding box); a scene obstacle needs the grid
+ * to extend past the box so that the band is whole on every side
+ * (sph_map_solid).  Triangles whose grown bounds miss the grid are
+ * skipped. */
+sph_status sph_boundary_map_generate_on(const sph_mesh* mesh,
+                                        const sph_boundary_map_config* cfg,
+                                        const int32_t dims[3],
+                                        const float lo[3], float* distance,
+                                        float* normal, uint8_t* present,
+                                        float* ms);
 /* BoundaryMap::Save / Load text format (boundary_map.cpp:161-279). */
 sph_status sph_boundary_map_save(const char* path,
                                  const sph_boundary_map_config* cfg,

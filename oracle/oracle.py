@@ -220,7 +220,7 @@ def scene_sdf(desc: SceneDesc, xyz: np.ndarray) -> np.ndarray:
 
 
 def struct_layout():
-    out = (C.c_size_t * 10)()
+    out = (C.c_size_t * 13)()
     lib.orc_struct_layout(out)
     return list(out)
 

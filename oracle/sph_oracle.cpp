@@ -1092,5 +1092,8 @@ void orc_struct_layout(size_t* out) {
     out[7] = offsetof(sph_scene, boundary);
     out[8] = offsetof(sph_scene_desc, sub_placement);
     out[9] = offsetof(sph_boundary, viscosity);
+    out[10] = sizeof(sph_map_solid);
+    out[11] = offsetof(sph_map_solid, distance);
+    out[12] = offsetof(sph_map_solid, solid_hi);
 }
 }

@@ -42,7 +42,8 @@ def test_struct_layout_matches_header():
     lay = O.struct_layout()
     mine = [C.sizeof(N.SphParams), C.sizeof(N.SphBoundary), C.sizeof(N.SphScene), C.sizeof(N.SceneDesc),
             C.sizeof(N.SphCounts), C.sizeof(N.SphSolid), N.SphParams.max_sub_particles.offset,
-            N.SphScene.boundary.offset, N.SceneDesc.sub_placement.offset, N.SphBoundary.viscosity.offset]
+            N.SphScene.boundary.offset, N.SceneDesc.sub_placement.offset, N.SphBoundary.viscosity.offset,
+            C.sizeof(N.SphMapSolid), N.SphMapSolid.distance.offset, N.SphMapSolid.solid_hi.offset]
     assert lay == mine
 
 
