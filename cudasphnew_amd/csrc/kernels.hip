@@ -568,14 +568,16 @@ __device__ __forceinline__ uint32_t near_bits(uint32_t far, int c) {
 }
 
 // One candidate of the density pass: poly6 sum and the "farther than h" bit.
-// d = h2 - r2 is negative exactly when r2 > h2 (IEEE subtraction keeps the
-// sign of the exact difference, and h2 - h2 = +0), so its sign bit is that
-// bit and one funnel shift per candidate builds the mask.
+// d = h2 - dz^2 - dy^2 - dx^2 as a chain of three fused multiply-adds (one
+// instruction fewer than forming r^2 and subtracting it).  Each step rounds
+// an exact result, which keeps its sign (and an exact zero is +0), so d's
+// sign bit is set exactly when this d is negative, i.e. when the candidate's
+// poly6 term is zero: the bit is that "farther than h" bit and one funnel
+// shift per candidate builds the mask.
 __device__ __forceinline__ void density_candidate(float xjx, float xjy, float xjz, float4 xi, float h2, float& sum,
                                                   uint32_t& far) {
     float dx = xjx - xi.x, dy = xjy - xi.y, dz = xjz - xi.z;
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float d = h2 - r2;
+    float d = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2)));
     far = __builtin_amdgcn_alignbit(far, __float_as_uint(d), 31);
     d = fmaxf(d, 0.0f);
     sum += d * d * d;
@@ -600,7 +602,7 @@ __device__ __forceinline__ uint64_t density_row(const At& at, int cnt, float4 xi
     for (int t = 64; t < cnt; ++t) {
         const float3 p = at(t);
         float dx = p.x - xi.x, dy = p.y - xi.y, dz = p.z - xi.z;
-        float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
+        float d = fmaxf(fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2))), 0.0f);
         sum += d * d * d;
     }
     return ((uint64_t)near_bits(far_hi, c64 - 32) << 32) | near_bits(far_lo, c32);
