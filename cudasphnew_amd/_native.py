@@ -267,6 +267,10 @@ SIGNATURES = [
     ("sph_example01_boundary", _st, [i3, f3, fptr, fptr, fptr, fptr, fptr]),
     ("sph_scene_grid_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
     ("sph_scene_sdf_samples", _st, [C.POINTER(SceneDesc), C.c_uint64, C.c_size_t, C.c_float, C.c_float, fptr]),
+    ("sph_scene_sdf_samples_maps", _st, [C.POINTER(SceneDesc), C.POINTER(SphMapSolid), C.c_int32, C.c_uint64,
+                                         C.c_size_t, C.c_float, C.c_float, fptr]),
+    ("sph_scene_grid_samples_maps", _st, [C.POINTER(SceneDesc), C.POINTER(SphMapSolid), C.c_int32, C.c_uint64,
+                                          C.c_size_t, C.c_float, C.c_float, fptr]),
     ("sph_mlp_default_config", _st, [f3, f3, C.POINTER(SphMlpConfig)]),
     ("sph_mlp_create", _st, [C.POINTER(SphMlpConfig), C.POINTER(_vp)]),
     ("sph_mlp_destroy", _st, [_vp]),

@@ -409,6 +409,24 @@ bool map_ok(const sph_map_solid& m, float support, float ghost_spacing) {
     return (size_t)m.dims[0] * m.dims[1] * m.dims[2] <= ((size_t)1 << 31);
 }
 
+sph_status prepare_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                               std::vector<MapSolid>& out) {
+    if (num_maps < 0 || (num_maps > 0 && !maps)) return SPH_INVALID_ARG;
+    out.resize((size_t)num_maps);
+    for (int32_t k = 0; k < num_maps; ++k) {
+        if (!map_ok(maps[k], d->boundary_support, ghost_spacing_of(*d))) return SPH_INVALID_ARG;
+        out[k].prepare(&maps[k]);
+    }
+    return SPH_OK;
+}
+
+// Union of the scene's solids and the map solids, > 0 inside a solid.
+float union_sdf(const sph_scene_desc& d, const std::vector<MapSolid>& maps, const float p[3]) {
+    float v = scene_sdf_point(d, p);
+    for (const MapSolid& m : maps) v = std::max(v, m.sdf(p));
+    return v;
+}
+
 bool finite3(const float* v) {
     return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]);
 }
@@ -677,8 +695,8 @@ sph_status sph_scene_sdf(const sph_scene_desc* d, const float* xyz, size_t n, fl
     return SPH_OK;
 }
 
-sph_status sph_scene_sdf_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
-                                 float near_width, float* xyzd) {
+static sph_status sdf_samples(const sph_scene_desc* d, const std::vector<MapSolid>& maps, uint64_t seed, size_t n,
+                              float near_frac, float near_width, float* xyzd) {
     if (!d || (n && !xyzd) || !(near_frac >= 0.0f && near_frac <= 1.0f) || !(near_width > 0.0f))
         return SPH_INVALID_ARG;
     for (int a = 0; a < 3; ++a)
@@ -702,7 +720,7 @@ sph_status sph_scene_sdf_samples(const sph_scene_desc* d, uint64_t seed, size_t 
         float p[3], v;
         for (;;) {
             draw(p);
-            v = scene_sdf_point(*d, p);
+            v = union_sdf(*d, maps, p);
             if (i >= n_near || std::fabs(v) < near_width) break;
             if (++tries > max_tries) return SPH_INVALID_ARG;   // band too thin to sample
         }
@@ -741,9 +759,9 @@ static float distance_texture(const float* g, const int dims[3], const float o[3
     return lerp(lerp(c00, c10, w[1]), lerp(c01, c11, w[1]), w[2]);
 }
 
-sph_status sph_scene_grid_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
-                                  float near_width, float* xyzd) {
-    sph_status st = sph_scene_sdf_samples(d, seed, n, near_frac, near_width, xyzd);
+static sph_status grid_samples(const sph_scene_desc* d, const std::vector<MapSolid>& maps, uint64_t seed, size_t n,
+                               float near_frac, float near_width, float* xyzd) {
+    sph_status st = sdf_samples(d, maps, seed, n, near_frac, near_width, xyzd);
     if (st != SPH_OK || n == 0) return st;
     const int dims[3] = {d->boundary_dims[0], d->boundary_dims[1], d->boundary_dims[2]};
     std::vector<float> g((size_t)dims[0] * dims[1] * dims[2]);
@@ -756,11 +774,39 @@ sph_status sph_scene_grid_samples(const sph_scene_desc* d, uint64_t seed, size_t
     b.rest_distance = d->rest_distance;
     b.support = d->boundary_support;
     b.reset();
-    b.set_distances([&](const float p[3]) { return scene_sdf_point(*d, p); });
+    b.set_distances([&](const float p[3]) { return union_sdf(*d, maps, p); });
     b.flatten(g.data(), nullptr, nullptr);
     for (size_t i = 0; i < n; ++i) xyzd[4 * i + 3] = distance_texture(g.data(), dims, d->boundary_origin,
                                                                       d->boundary_spacing, xyzd + 4 * i);
     return SPH_OK;
+}
+
+sph_status sph_scene_sdf_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
+                                 float near_width, float* xyzd) {
+    return sdf_samples(d, {}, seed, n, near_frac, near_width, xyzd);
+}
+
+sph_status sph_scene_grid_samples(const sph_scene_desc* d, uint64_t seed, size_t n, float near_frac,
+                                  float near_width, float* xyzd) {
+    return grid_samples(d, {}, seed, n, near_frac, near_width, xyzd);
+}
+
+sph_status sph_scene_sdf_samples_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                                      uint64_t seed, size_t n, float near_frac, float near_width, float* xyzd) {
+    if (!d) return SPH_INVALID_ARG;
+    std::vector<MapSolid> ms;
+    sph_status st = prepare_maps(d, maps, num_maps, ms);
+    if (st != SPH_OK) return st;
+    return sdf_samples(d, ms, seed, n, near_frac, near_width, xyzd);
+}
+
+sph_status sph_scene_grid_samples_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
+                                       uint64_t seed, size_t n, float near_frac, float near_width, float* xyzd) {
+    if (!d) return SPH_INVALID_ARG;
+    std::vector<MapSolid> ms;
+    sph_status st = prepare_maps(d, maps, num_maps, ms);
+    if (st != SPH_OK) return st;
+    return grid_samples(d, ms, seed, n, near_frac, near_width, xyzd);
 }
 
 // The three grids of a scene whose solids are the container, the analytic
@@ -776,11 +822,7 @@ static sph_status build_boundary(const sph_scene_desc* d, const std::vector<MapS
     g.rest_distance = d->rest_distance;
     g.support = d->boundary_support;
     g.reset();
-    g.set_distances([&](const float p[3]) {
-        float v = scene_sdf_point(*d, p);
-        for (const MapSolid& m : maps) v = std::max(v, m.sdf(p));
-        return v;
-    });
+    g.set_distances([&](const float p[3]) { return union_sdf(*d, maps, p); });
     const float mass = particle_mass_of(*d);
     const float gsp = ghost_spacing_of(*d);
     float c[3], e[3];
@@ -807,17 +849,6 @@ static bool boundary_desc_ok(const sph_scene_desc* d) {
     return d->boundary_spacing > 0.0f && finite3(d->boundary_origin);
 }
 
-static sph_status prepare_maps(const sph_scene_desc* d, const sph_map_solid* maps, int32_t num_maps,
-                               std::vector<MapSolid>& out) {
-    if (num_maps < 0 || (num_maps > 0 && !maps)) return SPH_INVALID_ARG;
-    out.resize((size_t)num_maps);
-    for (int32_t k = 0; k < num_maps; ++k) {
-        if (!map_ok(maps[k], d->boundary_support, ghost_spacing_of(*d))) return SPH_INVALID_ARG;
-        out[k].prepare(&maps[k]);
-    }
-    return SPH_OK;
-}
-
 sph_status sph_scene_boundary(const sph_scene_desc* d, float* distance, float* density, float* viscosity) {
     if (!boundary_desc_ok(d)) return SPH_INVALID_ARG;
     return build_boundary(d, {}, distance, density, viscosity);
@@ -838,11 +869,7 @@ sph_status sph_scene_sdf_maps(const sph_scene_desc* d, const sph_map_solid* maps
     std::vector<MapSolid> ms;
     sph_status st = prepare_maps(d, maps, num_maps, ms);
     if (st != SPH_OK) return st;
-    for (size_t i = 0; i < n; ++i) {
-        float v = scene_sdf_point(*d, xyz + 3 * i);
-        for (const MapSolid& m : ms) v = std::max(v, m.sdf(xyz + 3 * i));
-        out[i] = v;
-    }
+    for (size_t i = 0; i < n; ++i) out[i] = union_sdf(*d, ms, xyz + 3 * i);
     return SPH_OK;
 }
 

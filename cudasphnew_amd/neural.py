@@ -29,33 +29,37 @@ def boundary_box(desc: SceneDesc):
     return lo, (lo + ext).astype(np.float32)
 
 
-def sdf_samples(desc: SceneDesc, n: int, near_frac: float = 0.3, near_width: float | None = None,
-                seed: int = 1) -> np.ndarray:
-    """(n, 4) float32 (x, y, z, sdf) samples of the scene's boundary: a
-    fraction near_frac within near_width (default 3 h) of the surface, the
-    rest uniform over the boundary grid's box."""
+def _samples(plain_fn, maps_fn, desc, n, near_frac, near_width, seed, maps):
     if near_width is None:
         near_width = 3.0 * float(desc.h)
     out = np.empty((int(n), 4), np.float32)
-    st = lib.sph_scene_sdf_samples(C.byref(desc), int(seed), int(n), float(near_frac), float(near_width),
-                                   as_fptr(out))
+    if not maps:
+        st = plain_fn(C.byref(desc), int(seed), int(n), float(near_frac), float(near_width), as_fptr(out))
+    else:
+        from .scenes import _maps
+        arr, nm, keep = _maps(desc, maps)
+        st = maps_fn(C.byref(desc), arr, nm, int(seed), int(n), float(near_frac), float(near_width), as_fptr(out))
     if st != Status.OK:
         raise SphError(st, "sample generation failed")
     return out
+
+
+def sdf_samples(desc: SceneDesc, n: int, near_frac: float = 0.3, near_width: float | None = None,
+                seed: int = 1, maps=None) -> np.ndarray:
+    """(n, 4) float32 (x, y, z, sdf) samples of the scene's boundary: a
+    fraction near_frac within near_width (default 3 h) of the surface, the
+    rest uniform over the boundary grid's box.  `maps`: obstacles given as
+    boundary maps (scenes.mesh_obstacle), as for scenes.boundary_of."""
+    return _samples(lib.sph_scene_sdf_samples, lib.sph_scene_sdf_samples_maps, desc, n, near_frac, near_width,
+                    seed, maps)
 
 
 def grid_samples(desc: SceneDesc, n: int, near_frac: float = 0.3, near_width: float | None = None,
-                 seed: int = 1) -> np.ndarray:
+                 seed: int = 1, maps=None) -> np.ndarray:
     """The points of sdf_samples labelled with the boundary grid's distance
     texture (the field the step samples)."""
-    if near_width is None:
-        near_width = 3.0 * float(desc.h)
-    out = np.empty((int(n), 4), np.float32)
-    st = lib.sph_scene_grid_samples(C.byref(desc), int(seed), int(n), float(near_frac), float(near_width),
-                                    as_fptr(out))
-    if st != Status.OK:
-        raise SphError(st, "sample generation failed")
-    return out
+    return _samples(lib.sph_scene_grid_samples, lib.sph_scene_grid_samples_maps, desc, n, near_frac, near_width,
+                    seed, maps)
 
 
 class NeuralSDF:
@@ -153,22 +157,23 @@ def default_neural_config(desc: SceneDesc, tolerance_h: float = 0.05, band_h: fl
     return cfg
 
 
-def validation_samples(desc: SceneDesc, n: int = 1 << 20, seed: int = 1234) -> np.ndarray:
+def validation_samples(desc: SceneDesc, n: int = 1 << 20, seed: int = 1234, maps=None) -> np.ndarray:
     """Held-out grid-labelled samples: 80% within 3 h of the surface, 20%
     uniform."""
-    return grid_samples(desc, n, near_frac=0.8, near_width=3.0 * float(desc.h), seed=seed)
+    return grid_samples(desc, n, near_frac=0.8, near_width=3.0 * float(desc.h), seed=seed, maps=maps)
 
 
 def report_dict(r: SphNeuralReport) -> dict:
     return {f: (float(getattr(r, f)) if "error" in f else int(getattr(r, f))) for f, _ in SphNeuralReport._fields_}
 
 
-def attach(target, net: NeuralSDF, desc: SceneDesc, scene: int = 0, val=None, cfg=None) -> dict:
+def attach(target, net: NeuralSDF, desc: SceneDesc, scene: int = 0, val=None, cfg=None, maps=None) -> dict:
     """Attaches the network as the distance source of a ParticleSimulation
-    (or of scene `scene` of a Batch); returns the attach report."""
+    (or of scene `scene` of a Batch); returns the attach report.  `maps`:
+    the scene's map obstacles, for the default validation samples."""
     from .simulation import Batch
     if val is None:
-        val = validation_samples(desc)
+        val = validation_samples(desc, maps=maps)
     val = np.ascontiguousarray(val, np.float32).reshape(-1, 4)
     cfg = cfg or default_neural_config(desc)
     rep = SphNeuralReport()
@@ -189,15 +194,15 @@ def attach(target, net: NeuralSDF, desc: SceneDesc, scene: int = 0, val=None, cf
 
 
 def train_for_scene(desc: SceneDesc, steps: int = 20000, batch: int = 65536, n_samples: int = 1 << 22,
-                    seed: int = 1) -> NeuralSDF:
+                    seed: int = 1, maps=None) -> NeuralSDF:
     """The configuration's training recipe: 4M samples of the boundary grid's
     distance texture, 70% uniform / 30% within 3 h of the surface; output
     scaled by r0 (the texture's range); Adam lr 1e-3 with cosine decay to
-    1e-5."""
+    1e-5.  `maps`: the scene's map obstacles (scenes.mesh_obstacle)."""
     h = float(desc.h)
     net = NeuralSDF.for_scene(desc, seed=seed, out_scale=float(desc.rest_distance), lr=1e-3, lr_min=1e-5,
                               decay_steps=steps)
-    net.set_samples(grid_samples(desc, n_samples, near_frac=0.3, near_width=3.0 * h, seed=seed + 1))
+    net.set_samples(grid_samples(desc, n_samples, near_frac=0.3, near_width=3.0 * h, seed=seed + 1, maps=maps))
     net.train(steps, batch)
     return net
 

@@ -475,10 +475,6 @@ sph_status sph_scene_boundary_maps(const sph_scene_desc* desc,
                                    const sph_map_solid* maps, int32_t num_maps,
                                    float* distance, float* density,
                                    float* viscosity);
-/* sph_scene_sdf with the map solids in the union. */
-sph_status sph_scene_sdf_maps(const sph_scene_desc* desc,
-                              const sph_map_solid* maps, int32_t num_maps,
-                              const float* xyz, size_t n, float* out);
 
 /* Example01's own grid builder call (cu:2592-2612): box of half extents
  * (0.7,0.5,0.3) on the grid (-0.8,-0.6,-0.4)..(0.8,0.6,0.4), spacing h/4.
@@ -503,6 +499,24 @@ sph_status sph_scene_sdf_samples(const sph_scene_desc* desc, uint64_t seed,
 sph_status sph_scene_grid_samples(const sph_scene_desc* desc, uint64_t seed,
                                   size_t n, float near_frac, float near_width,
                                   float* xyzd);
+
+/* sph_scene_sdf with the map solids in the union. */
+sph_status sph_scene_sdf_maps(const sph_scene_desc* desc,
+                              const sph_map_solid* maps, int32_t num_maps,
+                              const float* xyz, size_t n, float* out);
+/* sph_scene_sdf_samples / sph_scene_grid_samples with the map solids in
+ * the union (the training and validation samples of a learned boundary for
+ * a scene whose obstacles come from meshes). */
+sph_status sph_scene_sdf_samples_maps(const sph_scene_desc* desc,
+                                      const sph_map_solid* maps,
+                                      int32_t num_maps, uint64_t seed,
+                                      size_t n, float near_frac,
+                                      float near_width, float* xyzd);
+sph_status sph_scene_grid_samples_maps(const sph_scene_desc* desc,
+                                       const sph_map_solid* maps,
+                                       int32_t num_maps, uint64_t seed,
+                                       size_t n, float near_frac,
+                                       float near_width, float* xyzd);
 
 /* ---- learned boundary SDF ------------------------------------------------
  * No reference counterpart: a compact stand-in for the distance texture

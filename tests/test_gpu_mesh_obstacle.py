@@ -8,14 +8,16 @@ test_gpu_mesh.py; the grids of a finely subdivided icosphere against those
 of the analytic sphere it approximates (polyhedral error <= 4e-4 for the
 radius below, trilinear error ~1e-4); particles of a dam break that runs
 into the mesh sphere stay out of it as far as they stay out of the analytic
-one."""
+one, also with the learned boundary trained on the scene with the map."""
 import ctypes as C
 import os
 
 import numpy as np
 import pytest
 
+from cudasphnew_amd import Field
 from cudasphnew_amd import meshsdf as M
+from cudasphnew_amd import neural as NN
 from cudasphnew_amd import scenes as S
 from cudasphnew_amd._native import SphSolid
 from cudasphnew_amd.simulation import ParticleSimulation
@@ -135,6 +137,26 @@ def test_mesh_sphere_grids_match_the_analytic_sphere():
         assert rel <= 0.05
 
 
+def run_against_sphere(scene, net=None, val=None):
+    """6,000 steps of 1e-4 s (the dam's front runs over the sphere): closest
+    approach of a particle to the sphere's centre, the largest number of
+    particles within 1.5 h of its surface at a check, the attach report and
+    the boundary sources of the last step."""
+    h = float(scene.desc.h)
+    with ParticleSimulation(scene).Init() as sim:
+        rep = sim.attach_neural(net, val=val) if net is not None else None
+        closest, touching = np.inf, 0
+        for _ in range(20):
+            sim.Advance(300)
+            pos = sim.positions()
+            assert np.all(np.isfinite(pos))
+            r = np.linalg.norm(pos - np.array(CENTER, np.float32), axis=1)
+            closest = min(closest, float(r.min()))
+            touching = max(touching, int((r < RADIUS + 1.5 * h).sum()))
+        src = np.bincount(sim.read(Field.WALL_SOURCE), minlength=3)
+    return closest, touching, rep, src
+
+
 def test_dam_break_around_a_mesh_obstacle():
     plain = S.config_desc("dam16k")
     bm, _, _ = mesh_obstacle(plain)
@@ -143,23 +165,39 @@ def test_dam_break_around_a_mesh_obstacle():
         scene = S.scene_from_desc(desc, maps)
         r0 = np.linalg.norm(scene.positions - np.array(CENTER, np.float32), axis=1)
         assert r0.min() > RADIUS + 0.02
-        with ParticleSimulation(scene).Init() as sim:
-            closest, touching = np.inf, 0
-            for _ in range(20):       # 6000 steps of 1e-4 s: the front runs over the sphere
-                sim.Advance(300)
-                pos = sim.positions()
-                assert np.all(np.isfinite(pos))
-                r = np.linalg.norm(pos - np.array(CENTER, np.float32), axis=1)
-                closest = min(closest, float(r.min()))
-                touching = max(touching, int((r < RADIUS + 1.5 * plain.h).sum()))
-            runs[name] = (closest, RADIUS - closest, touching)
-        print(f"{name}: closest approach to the centre {closest:.4f} (radius {RADIUS}), "
-              f"up to {touching} particles within 1.5 h of the surface")
+        runs[name] = run_against_sphere(scene)[:2]
+        print(f"{name}: closest approach to the centre {runs[name][0]:.4f} (radius {RADIUS}), "
+              f"up to {runs[name][1]} particles within 1.5 h of the surface")
     # the fluid runs against the sphere in both runs (the boundary force
     # holds particles about r0 = h off a wall, as off the container's walls)
     for name in runs:
-        assert runs[name][2] > 50
+        assert runs[name][1] > 50
         assert runs[name][0] < RADIUS + 1.25 * plain.h
     # and keeps off the mesh sphere as it keeps off the analytic one
     assert abs(runs["mesh"][0] - runs["analytic"][0]) < 2e-3
     assert runs["mesh"][0] > RADIUS
+
+
+def test_learned_boundary_around_a_mesh_obstacle():
+    """The learned boundary of a scene whose obstacle comes from a mesh:
+    trained on the scene's grid samples with the map (the recipe of the
+    plain scenes), validated within 0.05 h on its band, and the dam break
+    run with it keeps off the sphere as the grid run does."""
+    plain = S.config_desc("dam16k")
+    bm, _, _ = mesh_obstacle(plain)
+    net = NN.train_for_scene(plain, maps=[bm])
+    val = NN.validation_samples(plain, maps=[bm])
+    scene = S.scene_from_desc(plain, [bm])
+    grid = run_against_sphere(scene)
+    learned = run_against_sphere(scene, net, val)
+    rep, src = learned[2], learned[3]
+    print(f"grid: closest {grid[0]:.4f}, touching {grid[1]}; learned: closest {learned[0]:.4f}, "
+          f"touching {learned[1]}; report {rep}; sources {src}")
+    assert rep["mean_error"] < 0.05 * plain.h
+    assert rep["bricks_fallback"] <= 0.05 * rep["bricks_checked"]
+    assert src[1] > 0.2 * scene.n          # the network is the wall source of most wall particles
+    assert learned[1] > 50 and learned[0] > RADIUS
+    # the learned normals and distances differ from the grid's inside the
+    # penalty zone: the fluid keeps the same distance to 0.15 h (measured
+    # 0.07 h)
+    assert abs(learned[0] - grid[0]) < 0.15 * plain.h

@@ -142,3 +142,27 @@ def test_map_solid_rejects_a_thin_band_or_a_small_grid():
     bad = SphMapSolid()
     assert lib.sph_scene_sdf_maps(C.byref(plain), C.byref(bad), 1, None, 0, None) != 0
     assert lib.sph_scene_sdf_maps(C.byref(plain), None, 1, None, 0, None) != 0
+
+
+def test_samples_with_maps():
+    from cudasphnew_amd import neural as NN
+    plain = S.config_desc("dam16k")
+    bm = sphere_map(plain)
+    # uniform points do not depend on the field: same points, labels of the
+    # sphere's field up to the trilinear error
+    a = NN.grid_samples(with_sphere(), 20000, near_frac=0.0, seed=5)
+    b = NN.grid_samples(plain, 20000, near_frac=0.0, seed=5, maps=[bm])
+    np.testing.assert_array_equal(a[:, :3], b[:, :3])
+    off = np.abs(a[:, 3] - b[:, 3]) > 2e-4
+    assert off.mean() < 1e-3
+    assert np.abs(a[:, 3] - b[:, 3]).max() <= plain.rest_distance * 2 + 1e-6
+    # the map sphere's band is sampled, labelled with the union's distance
+    s = NN.sdf_samples(plain, 5000, near_frac=1.0, near_width=0.01, seed=7, maps=[bm])
+    assert np.all(np.abs(s[:, 3]) < 0.01)
+    np.testing.assert_array_equal(s[:, 3], S.scene_sdf(plain, s[:, :3], [bm]))
+    r = np.linalg.norm(s[:, :3] - np.array(CENTER), axis=1)
+    # the sphere's share of the band: its area against the box's, ~5%
+    assert (np.abs(r - RADIUS) < 0.011).mean() > 0.03
+    # a map the scene cannot use is refused here too
+    with pytest.raises(SphError):
+        NN.grid_samples(plain, 10, maps=[sphere_map(plain, band=0.01)])
