@@ -90,10 +90,12 @@ struct Engine {
     hipEvent_t next_event();
     sph_status resolve_timing();
 
-    // graph capture of the steady-state step
+    // graph capture of the steady-state step, plain and two-scale
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph_ts = nullptr;
+    hipGraphExec_t graph_exec_ts = nullptr;
 
     TwoScale* ts = nullptr;
     bool last_two_scale = false;
@@ -138,6 +140,9 @@ sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
 sph_status twoscale_counts(Engine& e, sph_counts* out);
 size_t twoscale_bytes(const Engine& e);
 uint8_t* twoscale_final_state(Engine& e);
+// True once the sub-particle pass has run: its launches are then the same
+// every step (steady state), so the two-scale step can be captured.
+bool twoscale_steady(const Engine& e);
 // Sub-particle positions of the last step by sub id into out (8N float4),
 // and the default / active sub id lists and counts into the view.
 sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view);

@@ -393,21 +393,28 @@ sph_status Engine::step(int n, bool two_scale) {
     if (two_scale && !ts) {
         sph_status st = twoscale_create(*this);
         if (st != SPH_OK) return st;
+        drop_graph();   // the step's buffers now include the sub-particle pass's
     }
     for (int i = 0; i < n; ++i) {
-        bool graphable = use_graph && !timing && keys_valid && cells_valid && !two_scale;
+        // steady state: every buffer pointer and launch of the step is fixed,
+        // so the step is captured once and replayed (a plain and a two-scale
+        // graph, each captured on its first steady-state step)
+        bool graphable = use_graph && !timing && keys_valid && cells_valid && (!two_scale || twoscale_steady(*this));
         if (graphable) {
-            if (!graph_exec) {
+            hipGraph_t& g = two_scale ? graph_ts : graph;
+            hipGraphExec_t& gx = two_scale ? graph_exec_ts : graph_exec;
+            if (!gx) {
                 if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                     return fail(SPH_DEVICE_ERROR, "hipStreamBeginCapture failed");
-                enqueue_step(false);
-                if (hipStreamEndCapture(stream, &graph) != hipSuccess)
+                enqueue_step(two_scale);
+                if (hipStreamEndCapture(stream, &g) != hipSuccess)
                     return fail(SPH_DEVICE_ERROR, "hipStreamEndCapture failed");
-                if (hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) != hipSuccess)
+                if (hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess)
                     return fail(SPH_DEVICE_ERROR, "hipGraphInstantiate failed");
             }
-            if (hipGraphLaunch(graph_exec, stream) != hipSuccess)
+            if (hipGraphLaunch(gx, stream) != hipSuccess)
                 return fail(SPH_DEVICE_ERROR, "hipGraphLaunch failed");
+            last_two_scale = two_scale;
         } else {
             enqueue_step(two_scale);
             // bound the number of outstanding timing events
@@ -471,8 +478,10 @@ sph_status Engine::upload_params(int s, const sph_params& p) {
 void Engine::drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
-    graph_exec = nullptr;
-    graph = nullptr;
+    if (graph_exec_ts) (void)hipGraphExecDestroy(graph_exec_ts);
+    if (graph_ts) (void)hipGraphDestroy(graph_ts);
+    graph_exec = graph_exec_ts = nullptr;
+    graph = graph_ts = nullptr;
 }
 
 static bool lanes_ok(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16; }

@@ -684,6 +684,8 @@ size_t twoscale_bytes(const Engine& e) { return e.ts ? e.ts->bytes : 0; }
 
 uint8_t* twoscale_final_state(Engine& e) { return e.ts->state_fin; }
 
+bool twoscale_steady(const Engine& e) { return e.ts && e.ts->sub_cells_valid; }
+
 sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t nbytes) {
     TwoScale& t = *e.ts;
     TSCounts c;
