@@ -23,6 +23,7 @@
 //            667-696, 836-930).
 // Every kernel reads the live counts from device memory; grids are sized
 // for the sub-particle capacity.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -391,42 +392,74 @@ __device__ __forceinline__ void sub_range(const DevScene& S, float4 x, float h, 
     lz = range_coord(x.z, -h, S.oz, S.cell_sub, S.snz); hz = range_coord(x.z, h, S.oz, S.cell_sub, S.snz);
 }
 
+// The sub density pass gives each sub-particle kTsLanes consecutive
+// lanes: lane j of a group takes candidates b + j, b + j + 4, ...
+// of every row run and the group adds its partial sums by shuffles.  The
+// sub population (~10^5) is too small to hide memory latency at one thread
+// per particle.  A fixed grid strides over nsub * kTsLanes lanes (the count
+// is read on the device); the stride is a multiple of kTsLanes, so the lanes
+// of a group stay together.
+constexpr int kTsLanes = 4;
+constexpr unsigned kTsGrid = 2048;
+
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int m = 1; m < kTsLanes; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 // compute_sub_particle_density_pressure (cu:259-313): split subs only, no
 // wall term.
 __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                    const float4* __restrict__ spos_s, float4* __restrict__ svel_s,
                                                    const int2* __restrict__ srange,
                                                    const DevScene* __restrict__ scenes) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt->nsub) return;
-    if (sperm[t] >= 8u * cnt->nsplit) return;   // boundary sub: projected density
+    const uint64_t total = (uint64_t)cnt->nsub * kTsLanes;
+    const uint32_t nsplit8 = 8u * cnt->nsplit;
     const DevScene& S = scenes[0];
-    float4 xi = spos_s[t];
     const float h2 = S.h2_sub;
-    int lx, hx, ly, hy, lz, hz;
-    sub_range(S, xi, S.h_sub, lx, hx, ly, hy, lz, hz);
-    float sum = 0.0f;
-    for (int k = lz; k <= hz; ++k)
-        for (int j = ly; j <= hy; ++j) {
-            uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
-            int b = 0x7fffffff, e = -1;
-            for (int i = lx; i <= hx; ++i) {
-                int2 r = srange[row + (uint32_t)i];
-                if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
-            }
-            for (int q = b; q < e; ++q) {
-                float4 xj = spos_s[q];
-                float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
-                sum += d * d * d;
-            }
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const uint32_t t = (uint32_t)(g / kTsLanes);
+        const int lane = (int)(g % kTsLanes);
+        const bool split = sperm[t] < nsplit8;   // boundary subs: projected density
+        float sum = 0.0f;
+        if (split) {
+            const float4 xi = spos_s[t];
+            int lx, hx, ly, hy, lz, hz;
+            sub_range(S, xi, S.h_sub, lx, hx, ly, hy, lz, hz);
+            for (int k = lz; k <= hz; ++k)
+                for (int j = ly; j <= hy; ++j) {
+                    uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
+                    int b = 0x7fffffff, e = -1;
+                    for (int i = lx; i <= hx; ++i) {
+                        int2 r = srange[row + (uint32_t)i];
+                        if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
+                    }
+                    // (an empty row leaves b = INT_MAX: b + lane must not be formed)
+                    if (b >= e) continue;
+                    for (int q = b + lane; q < e; q += kTsLanes) {
+                        float4 xj = spos_s[q];
+                        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                        float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
+                        sum += d * d * d;
+                    }
+                }
         }
-    reinterpret_cast<float*>(svel_s)[4 * (size_t)t + 3] = sum * (S.poly6_sub * S.mass_sub);
+        sum = group_sum(sum);   // every lane of the group takes part
+        if (split && lane == 0) reinterpret_cast<float*>(svel_s)[4 * (size_t)t + 3] = sum * (S.poly6_sub * S.mass_sub);
+    }
 }
 
 // compute_sub_particle_acceleration (cu:477-568) for split subs (boundary
 // subs: zero acceleration), then integrate_sub_particles_euler (cu:667-696)
-// and the collision response for every sub-particle.
+// and the collision response for every sub-particle.  One thread per
+// sub-particle.  Each row run is first reduced to a 64-bit mask of the
+// candidates within h (cheap: the distance test only), and the pair terms
+// then run over the set bits: a wavefront executes the pair terms as often
+// as its lane with the most neighbours in the row, not once per candidate
+// that any lane hits.  Rows over 64 candidates take the direct loop for the
+// rest.  The terms are added in the same candidate order as a direct loop.
 __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                  const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
                                                  const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
@@ -447,6 +480,30 @@ __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cn
         int lx, hx, ly, hy, lz, hz;
         sub_range(S, xi, h, lx, hx, ly, hy, lz, hz);
         float fpx = 0, fpy = 0, fpz = 0, fvx = 0, fvy = 0, fvz = 0, cgx = 0, cgy = 0, cgz = 0, cl = 0;
+        auto pair = [&](int q) {
+            float4 xj = spos_s[q];
+            float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float4 vj = svel_s[q];
+            float inv_r = __builtin_amdgcn_rsqf(r2);
+            float rn = r2 * inv_r;
+            float invj = __builtin_amdgcn_rcpf(vj.w);
+            float pj = S.stiff * vj.w - kr0;
+            float hr = h - rn, qd = h2 - r2;
+            float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
+            fpx += cp * dx; fpy += cp * dy; fpz += cp * dz;
+            float wv = invj * hr;
+            fvx += wv * (vj.x - vi.x); fvy += wv * (vj.y - vi.y); fvz += wv * (vj.z - vi.z);
+            float wc = invj * qd * qd;
+            cgx += wc * dx; cgy += wc * dy; cgz += wc * dz;
+            cl += invj * qd * (3.0f * h2 - 7.0f * r2);
+        };
+        auto near = [&](int q) {
+            const float4 xj = spos_s[q];
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            return r2 <= h2 && r2 > 0.0f;
+        };
         for (int k = lz; k <= hz; ++k)
             for (int j = ly; j <= hy; ++j) {
                 uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
@@ -455,26 +512,18 @@ __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cn
                     int2 r = srange[row + (uint32_t)i];
                     if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
                 }
-                for (int q = b; q < e; ++q) {
-                    float4 xj = spos_s[q];
-                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    float r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 <= h2 && r2 > 0.0f) {
-                        float4 vj = svel_s[q];
-                        float inv_r = __builtin_amdgcn_rsqf(r2);
-                        float rn = r2 * inv_r;
-                        float invj = __builtin_amdgcn_rcpf(vj.w);
-                        float pj = S.stiff * vj.w - kr0;
-                        float hr = h - rn, qd = h2 - r2;
-                        float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
-                        fpx += cp * dx; fpy += cp * dy; fpz += cp * dz;
-                        float wv = invj * hr;
-                        fvx += wv * (vj.x - vi.x); fvy += wv * (vj.y - vi.y); fvz += wv * (vj.z - vi.z);
-                        float wc = invj * qd * qd;
-                        cgx += wc * dx; cgy += wc * dy; cgz += wc * dz;
-                        cl += invj * qd * (3.0f * h2 - 7.0f * r2);
-                    }
+                if (b >= e) continue;   // empty row
+                const int c64 = min(e - b, 64);
+                uint64_t m = 0;
+                for (int u = 0; u < c64; ++u)
+                    if (near(b + u)) m |= 1ull << u;
+                while (m) {
+                    const int u = __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    pair(b + u);
                 }
+                for (int q = b + 64; q < e; ++q)
+                    if (near(q)) pair(q);
             }
         const float cpf = -S.mass_sub * S.grad_spiky_sub, cvf = S.mu * S.mass_sub * S.lapl_visc_sub;
         float Fx = cvf * fvx + cpf * fpx, Fy = cvf * fvy + cpf * fpy, Fz = cvf * fvz + cpf * fpz;
@@ -641,7 +690,8 @@ void twoscale_enqueue(Engine& e) {
     }
     t.sub_cells_valid = true;
     e.mark(kStageTwoScaleSubSort);
-    hipLaunchKernelGGL(k_ts_density, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
+    const unsigned sub_grid = (unsigned)std::min<uint64_t>(kTsGrid, ((uint64_t)t.cap * kTsLanes + kB - 1) / kB);
+    hipLaunchKernelGGL(k_ts_density, dim3(sub_grid), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
                        t.srange, e.d_scenes);
     hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
                        t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
