@@ -5,8 +5,10 @@
 // pass reads coarse positions, velocities and densities before integration,
 // as the reference's order does; nothing feeds back into the coarse step):
 //   state    shift_state + find_split_particles + find_boundary_particles
-//            (cu:730-740, 932-1017); the boundary bit is pulled from split
-//            neighbours instead of the reference's racy byte OR.
+//            (cu:730-740, 932-1017); the boundary bit is pushed from the
+//            split particles as in the reference, but into a separate final
+//            state array: every writer of a byte stores the same value
+//            (state | 2) instead of the reference's racy byte OR.
 //   collect  check_split_boundary_default + 3 exclusive scans + collect_ids
 //            (cu:1092-1162, 2509-2567) as one 3-way compaction over
 //            original ids; counts stay on the device (the reference copies
@@ -70,9 +72,12 @@ constexpr int kCompactTile = kB * kCompactItems;
 
 inline unsigned nblocks(uint32_t n) { return (n + kB - 1) / kB; }
 
-// shift_state + find_split_particles, sorted order (own slot only).
+// shift_state + find_split_particles, sorted order (own slot only); the
+// final state starts as the shifted state (the boundary bit is set by
+// k_ts_mark).
 __global__ __launch_bounds__(kB) void k_ts_split(const float4* __restrict__ pos, uint8_t* __restrict__ state,
-                                                 uint32_t n, const DevScene* __restrict__ scenes) {
+                                                 uint8_t* __restrict__ state_fin, uint32_t n,
+                                                 const DevScene* __restrict__ scenes) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[0];
@@ -82,33 +87,41 @@ __global__ __launch_bounds__(kB) void k_ts_split(const float4* __restrict__ pos,
               x.z >= S.split_lo[2] && x.z <= S.split_hi[2];
     if (in) st |= 1;
     state[s] = st;
+    state_fin[s] = st;
 }
 
-// find_boundary_particles as a pull: bit 2 if any particle within h has its
-// split bit set this step.  Also records the class by original id.
-__global__ __launch_bounds__(kB) void k_ts_boundary(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
-                                                    const int2* __restrict__ range, const uint32_t* __restrict__ oid,
-                                                    uint32_t n, const DevScene* __restrict__ scenes,
-                                                    uint8_t* __restrict__ state_fin, uint8_t* __restrict__ cls) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// find_boundary_particles (bit 2: a particle with its split bit set this
+// step lies within h) as the reference's push from the split particles,
+// which are few: each sets the bit of every particle within h of it, itself
+// included.  Every writer of a byte writes the same value (the particle's
+// state | 2), so the order of the writes does not matter.
+// The split particles are few (~10^4): kMarkLanes lanes share each one's
+// candidates (lane l takes b + l, b + l + kMarkLanes, ... of every row run;
+// no lane needs another's result).  Measured against one thread per split
+// particle (49.5 us) and one row per lane (45 us) at 262,144 particles:
+// 39.5 us; the pull of the receiving particles took 46 us.
+constexpr int kMarkLanes = 8;
+
+__global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
+                                                const int2* __restrict__ range, uint32_t n,
+                                                const DevScene* __restrict__ scenes,
+                                                uint8_t* __restrict__ state_fin) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = (uint32_t)(g / kMarkLanes);
+    const int lane = (int)(g % kMarkLanes);
     if (s >= n) return;
+    const uint8_t ss = state[s];
+    if (!(ss & 1)) return;
+    if (lane == 0) state_fin[s] = (uint8_t)(ss | 2);   // within h of itself
     const DevScene& S = scenes[0];
-    float4 xi = pos[s];
+    const float4 xi = pos[s];
     const float h = S.h, h2 = S.h2;
-    int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
-    int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
-    int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
-    // split particles lie in the split box: a particle farther than h from
-    // the box has none within h (the 1e-5 margin covers the rounding)
-    const float bx = fmaxf(fmaxf(S.split_lo[0] - xi.x, xi.x - S.split_hi[0]), 0.0f);
-    const float by = fmaxf(fmaxf(S.split_lo[1] - xi.y, xi.y - S.split_hi[1]), 0.0f);
-    const float bz = fmaxf(fmaxf(S.split_lo[2] - xi.z, xi.z - S.split_hi[2]), 0.0f);
-    bool hit = false;
-    if ((bx * bx + by * by + bz * bz) * 0.99999f > h2) hz = lz - 1;   // no row to visit
-    // the cells of an x-row have consecutive keys, so their particles are
-    // one run of slots: 9 runs, the 3 range loads of a row issued together
-    for (int k = lz; k <= hz && !hit; ++k)
-        for (int j = ly; j <= hy && !hit; ++j) {
+    const int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
+    const int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
+    const int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+    // the cells of an x-row have consecutive keys: one run of slots per row
+    for (int k = lz; k <= hz; ++k)
+        for (int j = ly; j <= hy; ++j) {
             const uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
             int b = 0x7fffffff, e = -1;
             for (int i = lx; i <= hx; ++i) {
@@ -118,25 +131,28 @@ __global__ __launch_bounds__(kB) void k_ts_boundary(const float4* __restrict__ p
                     e = max(e, r.y);
                 }
             }
-            // split flags eight at a time: the loads are issued together
-            // instead of one latency per candidate
-            for (int q0 = b; q0 < e && !hit; q0 += 8) {
-                uint8_t f[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) f[u] = q0 + u < e ? state[q0 + u] : (uint8_t)0;
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    if (!(f[u] & 1)) continue;
-                    const float4 xj = pos[q0 + u];
-                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    if (dx * dx + dy * dy + dz * dz <= h2) hit = true;
-                }
+            if (b >= e) continue;   // empty row (b = INT_MAX: b + lane must not be formed)
+            for (int q = b + lane; q < e; q += kMarkLanes) {
+                // split particles mark themselves (above): the stores go to
+                // the few non-split particles of the box's shell only, not to
+                // the many bytes of the box that every neighbour would store
+                const uint8_t sq = state[q];
+                if (sq & 1) continue;
+                const float4 xj = pos[q];
+                const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                if (dx * dx + dy * dy + dz * dz <= h2) state_fin[q] = (uint8_t)(sq | 2);
             }
         }
-    uint8_t st = state[s];
-    if (hit) st |= 2;
-    state_fin[s] = st;
-    int c = st & 3;
+}
+
+// The class of each particle by original id: 0 default, 1 split (split bit,
+// boundary bit set by itself), 2 boundary only.
+__global__ __launch_bounds__(kB) void k_ts_classify(const uint8_t* __restrict__ state_fin,
+                                                    const uint32_t* __restrict__ oid, uint32_t n,
+                                                    uint8_t* __restrict__ cls) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int c = state_fin[s] & 3;
     cls[oid[s]] = c == 0 ? 0 : (c == 2 ? 2 : 1);
 }
 
@@ -661,9 +677,11 @@ void twoscale_enqueue(Engine& e) {
     TwoScale& t = *e.ts;
     hipStream_t st = e.stream;
     const uint32_t n = e.N;
-    hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, n, e.d_scenes);
-    hipLaunchKernelGGL(k_ts_boundary, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, e.range, e.oid_b, n,
-                       e.d_scenes, t.state_fin, t.cls);
+    hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, t.state_fin, n,
+                       e.d_scenes);
+    hipLaunchKernelGGL(k_ts_mark, dim3((unsigned)(((uint64_t)n * kMarkLanes + kB - 1) / kB)), dim3(kB), 0, st,
+                       e.pos_b, e.state_b, e.range, n, e.d_scenes, t.state_fin);
+    hipLaunchKernelGGL(k_ts_classify, dim3(nblocks(n)), dim3(kB), 0, st, t.state_fin, e.oid_b, n, t.cls);
     e.mark(kStageTwoScaleState);
     hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
     if (t.sub_cells_valid)
