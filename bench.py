@@ -246,6 +246,7 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
     sc2 = S.make_scene("two_scale")
     sim2 = S.ParticleSimulation(sc2).Init()
     sim2.Advance(1)
+    tun2 = sim2.autotune(reps=3)            # coarse density/force variants
     sim2.AdvanceTwoScale(max(a.warmup, 2))
     sim2.sync()
     # eager launches: the two-scale step's graph replay measured no faster
@@ -265,6 +266,7 @@ def bench_forms(S, a, plain_ms: float, sc) -> dict:
         "ms_per_step_graph": round(ms2_graph, 5), "ms_per_step_stage_events": round(ms2_staged, 5),
         "particles": sc2.n,
         "sub_particles": int(c2.num_sub), "split": int(c2.num_split), "boundary": int(c2.num_boundary),
+        "kernel_variants": tun2,
         "stage_ms": {k: round(v, 5) for k, v in st2.items() if v > 0},
         "note": "coarse particle-updates/s of AdvanceTwoScale (sub-particle pass included), eager launches; "
                 "ms_per_step_graph: the same steps as graph replays; stage times from a run with events",
@@ -352,6 +354,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-graph", action="store_true", help="time the 1M steps eagerly instead of as graph replays")
     ap.add_argument("--train-profile", type=int, default=0,
                     help="train the 1M scene's network for this many steps (batch 65536), report and exit")
+    ap.add_argument("--two-scale-profile", type=int, default=0,
+                    help="run this many two-scale steps of the 262,144-particle two-scale workload and exit "
+                         "(for kernel traces of the sub-particle pass)")
     ap.add_argument("--train-steps", type=int, default=20000, help="training steps of the 1M scene's network")
     ap.add_argument("--batch-train-steps", type=int, default=2000, help="training steps per batch scene network")
     ap.add_argument("--batch-steps", type=int, default=50, help="timed steps of the batched forms")
@@ -368,6 +373,17 @@ def main(argv=None) -> int:
 
     if a.train_profile:
         return train_profile(S, a)
+    if a.two_scale_profile:
+        sc2 = S.make_scene("two_scale")
+        sim2 = S.ParticleSimulation(sc2).Init()
+        sim2.Advance(1)
+        sim2.autotune(reps=3)
+        sim2.AdvanceTwoScale(a.two_scale_profile)
+        sim2.sync()
+        c2 = sim2.counts()
+        log(f"two-scale profile: {a.two_scale_profile} steps, {sc2.n} coarse, {c2.num_split} split, "
+            f"{c2.num_boundary} boundary, {c2.num_sub} sub-particles")
+        return 0
     if a.copy_probe:
         copy_probe(256 << 20)
     t0 = time.time()

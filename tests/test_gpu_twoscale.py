@@ -115,6 +115,23 @@ def test_two_scale_reference_slab_one_step(dam16k, placement):
     assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
 
 
+def test_two_scale_with_hit_masks(dam16k):
+    """The two-scale step with the hit-mask kernel pair for the coarse
+    physics (the autotuner's choice at large sizes) against the oracle."""
+    sc = dam16k.with_positions(dam16k.positions)
+    sim = S.ParticleSimulation(sc).Init()
+    sim.set_tuning(hit_masks=1, hit_list=48)
+    sim.AdvanceTwoScale(1)
+    sim.sync()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1, two_scale=True)
+    orc32 = O.OracleSim(sc, real=1, threads=THREADS)
+    orc32.step(1, two_scale=True)
+    c = _compare_two_scale(sim, orc, orc32)
+    assert c.num_split > 1000
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
+
+
 def test_two_scale_state_history_over_steps(dam16k):
     sc = dam16k.with_positions(dam16k.positions)
     sim = S.ParticleSimulation(sc).Init()
