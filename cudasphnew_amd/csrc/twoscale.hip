@@ -34,7 +34,9 @@
 namespace sph {
 
 struct TSCounts {
-    uint32_t nsplit, nboundary, ndefault, nsub, nparents, nsub_prev, pad0, pad1;
+    uint32_t nsplit, nboundary, ndefault, nsub, nparents, nsub_prev;
+    uint32_t nmark;   // entries of the split list of this step (k_ts_split); zeroed by k_ts_prev
+    uint32_t pad1;
 };
 
 struct TwoScale {
@@ -43,6 +45,7 @@ struct TwoScale {
     uint32_t sub_cells = 0;
     int sub_bits = 1;
     uint8_t* state_fin = nullptr;   // final state, sorted order
+    uint32_t* mark_list = nullptr;  // [N] slots of this step's split particles, in no order
     uint8_t* cls = nullptr;         // class by original id: 0 default, 1 split, 2 boundary
     uint32_t *split_ids = nullptr, *boundary_ids = nullptr, *default_ids = nullptr;
     uint32_t* blk = nullptr;        // [3][nblk] block counts -> offsets
@@ -74,20 +77,37 @@ inline unsigned nblocks(uint32_t n) { return (n + kB - 1) / kB; }
 
 // shift_state + find_split_particles, sorted order (own slot only); the
 // final state starts as the shifted state (the boundary bit is set by
-// k_ts_mark).
+// k_ts_mark), and the split particles' slots are appended to the split list
+// (one atomic per wavefront; the list order does not matter to k_ts_mark).
 __global__ __launch_bounds__(kB) void k_ts_split(const float4* __restrict__ pos, uint8_t* __restrict__ state,
                                                  uint8_t* __restrict__ state_fin, uint32_t n,
-                                                 const DevScene* __restrict__ scenes) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const DevScene& S = scenes[0];
-    float4 x = pos[s];
-    uint8_t st = (uint8_t)(state[s] << 2);
-    bool in = x.x >= S.split_lo[0] && x.x <= S.split_hi[0] && x.y >= S.split_lo[1] && x.y <= S.split_hi[1] &&
-              x.z >= S.split_lo[2] && x.z <= S.split_hi[2];
-    if (in) st |= 1;
-    state[s] = st;
-    state_fin[s] = st;
+                                                 const DevScene* __restrict__ scenes, TSCounts* __restrict__ cnt,
+                                                 uint32_t* __restrict__ mark_list) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if ((s & ~63u) >= n) return;   // the whole wavefront lies past the end
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    bool in = false;
+    if (valid) {
+        const DevScene& S = scenes[0];
+        float4 x = pos[s];
+        uint8_t st = (uint8_t)(state[s] << 2);
+        in = x.x >= S.split_lo[0] && x.x <= S.split_hi[0] && x.y >= S.split_lo[1] && x.y <= S.split_hi[1] &&
+             x.z >= S.split_lo[2] && x.z <= S.split_hi[2];
+        if (in) st |= 1;
+        state[s] = st;
+        state_fin[s] = st;
+    }
+    const uint64_t ins = __ballot(in);
+    if (!ins) return;
+    const int leader = __ffsll((unsigned long long)ins) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt->nmark, (uint32_t)__popcll(ins));
+    base = __shfl(base, leader, 64);
+    if (in) {
+        const uint64_t below = lane == 0 ? 0ull : (ins & (~0ull >> (64 - lane)));
+        mark_list[base + (uint32_t)__popcll(below)] = s;
+    }
 }
 
 // find_boundary_particles (bit 2: a particle with its split bit set this
@@ -95,33 +115,40 @@ __global__ __launch_bounds__(kB) void k_ts_split(const float4* __restrict__ pos,
 // which are few: each sets the bit of every particle within h of it, itself
 // included.  Every writer of a byte writes the same value (the particle's
 // state | 2), so the order of the writes does not matter.
-// The split particles are few (~10^4): kMarkLanes lanes share each one's
-// candidates (lane l takes b + l, b + l + kMarkLanes, ... of every row run;
-// no lane needs another's result).  Measured against one thread per split
-// particle (49.5 us) and one row per lane (45 us) at 262,144 particles:
-// 39.5 us; the pull of the receiving particles took 46 us.
-constexpr int kMarkLanes = 8;
+// One wavefront per split particle (a fixed grid of wavefronts strides over
+// the split list): lane l takes neighbour row l / 7 (nine rows as a rule;
+// rows past 8 in the rare four-cell spans go to lanes l / 7 + 9, ...) and
+// every seventh candidate of it from l % 7.  Each lane then has a short
+// chain of dependent loads (one range round, ~4 candidates); with one thread
+// per split particle the chain ran through all ~216 candidates and the pass
+// waited on memory (38 us at 262,144 particles with eight lanes each).
+constexpr int kMarkRowLanes = 7;
+constexpr unsigned kMarkGrid = 1024;   // blocks: 4,096 wavefronts
 
 __global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
-                                                const int2* __restrict__ range, uint32_t n,
-                                                const DevScene* __restrict__ scenes,
+                                                const int2* __restrict__ range, const DevScene* __restrict__ scenes,
+                                                const TSCounts* __restrict__ cnt,
+                                                const uint32_t* __restrict__ mark_list,
                                                 uint8_t* __restrict__ state_fin) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t s = (uint32_t)(g / kMarkLanes);
-    const int lane = (int)(g % kMarkLanes);
-    if (s >= n) return;
-    const uint8_t ss = state[s];
-    if (!(ss & 1)) return;
-    if (lane == 0) state_fin[s] = (uint8_t)(ss | 2);   // within h of itself
+    const uint32_t nmark = cnt->nmark;
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const DevScene& S = scenes[0];
-    const float4 xi = pos[s];
     const float h = S.h, h2 = S.h2;
-    const int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
-    const int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
-    const int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
-    // the cells of an x-row have consecutive keys: one run of slots per row
-    for (int k = lz; k <= hz; ++k)
-        for (int j = ly; j <= hy; ++j) {
+    for (uint32_t w = wave; w < nmark; w += nwaves) {
+        const uint32_t s = mark_list[w];
+        if (lane == 0) state_fin[s] = (uint8_t)(state[s] | 2);   // within h of itself
+        if (lane >= 9 * kMarkRowLanes) continue;
+        const float4 xi = pos[s];
+        const int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
+        const int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
+        const int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+        if (hy < ly || hz < lz) continue;   // a range wrapped by the grid's "% dim": no rows
+        const int ny_rows = hy - ly + 1, nrows = (hz - lz + 1) * ny_rows;
+        const int sub = lane % kMarkRowLanes;
+        for (int row = lane / kMarkRowLanes; row < nrows; row += 9) {
+            const int k = lz + row / ny_rows, j = ly + row % ny_rows;
             const uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
             int b = 0x7fffffff, e = -1;
             for (int i = lx; i <= hx; ++i) {
@@ -131,11 +158,10 @@ __global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, 
                     e = max(e, r.y);
                 }
             }
-            if (b >= e) continue;   // empty row (b = INT_MAX: b + lane must not be formed)
-            for (int q = b + lane; q < e; q += kMarkLanes) {
+            if (b >= e) continue;   // empty row (b = INT_MAX: b + sub must not be formed)
+            for (int q = b + sub; q < e; q += kMarkRowLanes) {
                 // split particles mark themselves (above): the stores go to
-                // the few non-split particles of the box's shell only, not to
-                // the many bytes of the box that every neighbour would store
+                // the few non-split particles of the box's shell only
                 const uint8_t sq = state[q];
                 if (sq & 1) continue;
                 const float4 xj = pos[q];
@@ -143,6 +169,7 @@ __global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, 
                 if (dx * dx + dy * dy + dz * dz <= h2) state_fin[q] = (uint8_t)(sq | 2);
             }
         }
+    }
 }
 
 // The class of each particle by original id: 0 default, 1 split (split bit,
@@ -587,7 +614,10 @@ __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cn
 }
 
 __global__ void k_ts_prev(TSCounts* __restrict__ cnt) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) cnt->nsub_prev = cnt->nsub;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        cnt->nsub_prev = cnt->nsub;
+        cnt->nmark = 0;   // the next step's split list starts empty
+    }
 }
 
 // Sub-particle positions of the last step by sub id, for a renderer:
@@ -631,6 +661,7 @@ sph_status twoscale_create(Engine& e) {
     t->sub_bits = 1;
     while (t->sub_bits < 32 && (1ull << t->sub_bits) < (uint64_t)t->sub_cells) ++t->sub_bits;
     bool ok = ts_alloc(*t, &t->state_fin, e.N) == hipSuccess && ts_alloc(*t, &t->cls, e.N) == hipSuccess &&
+              ts_alloc(*t, &t->mark_list, e.N) == hipSuccess &&
               ts_alloc(*t, &t->split_ids, e.N) == hipSuccess && ts_alloc(*t, &t->boundary_ids, e.N) == hipSuccess &&
               ts_alloc(*t, &t->default_ids, e.N) == hipSuccess &&
               ts_alloc(*t, &t->blk, 3 * (size_t)t->nblk) == hipSuccess && ts_alloc(*t, &t->counts, 1) == hipSuccess &&
@@ -664,7 +695,7 @@ void twoscale_destroy(Engine& e) {
     void* ptrs[] = {t->state_fin, t->cls,   t->split_ids, t->boundary_ids, t->default_ids, t->blk,
                     t->counts,    t->spos,  t->svel,      t->sid,          t->skey,        t->sskey,
                     t->sperm,     t->spos_s, t->svel_s,   t->spos_out,     t->svel_out,    t->sacc,
-                    t->srange,    t->sperm_raw};
+                    t->srange,    t->sperm_raw, t->mark_list};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t->sorter.release();
@@ -678,9 +709,9 @@ void twoscale_enqueue(Engine& e) {
     hipStream_t st = e.stream;
     const uint32_t n = e.N;
     hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, t.state_fin, n,
-                       e.d_scenes);
-    hipLaunchKernelGGL(k_ts_mark, dim3((unsigned)(((uint64_t)n * kMarkLanes + kB - 1) / kB)), dim3(kB), 0, st,
-                       e.pos_b, e.state_b, e.range, n, e.d_scenes, t.state_fin);
+                       e.d_scenes, t.counts, t.mark_list);
+    hipLaunchKernelGGL(k_ts_mark, dim3(kMarkGrid), dim3(kB), 0, st, e.pos_b, e.state_b, e.range, e.d_scenes,
+                       t.counts, t.mark_list, t.state_fin);
     hipLaunchKernelGGL(k_ts_classify, dim3(nblocks(n)), dim3(kB), 0, st, t.state_fin, e.oid_b, n, t.cls);
     e.mark(kStageTwoScaleState);
     hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
