@@ -481,6 +481,7 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
                     }
                     // (an empty row leaves b = INT_MAX: b + lane must not be formed)
                     if (b >= e) continue;
+#pragma unroll 4
                     for (int q = b + lane; q < e; q += kTsLanes) {
                         float4 xj = spos_s[q];
                         float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
@@ -558,6 +559,7 @@ __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cn
                 if (b >= e) continue;   // empty row
                 const int c64 = min(e - b, 64);
                 uint64_t m = 0;
+#pragma unroll 4
                 for (int u = 0; u < c64; ++u)
                     if (near(b + u)) m |= 1ull << u;
                 while (m) {
