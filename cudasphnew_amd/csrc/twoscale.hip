@@ -336,35 +336,56 @@ __global__ __launch_bounds__(kB) void k_ts_seed(const TSCounts* __restrict__ cnt
 // project_quantities (cu:570-622, 1328-1381): boundary sub-particles take the
 // unweighted mean density of the coarse particles within h_sub, searched on
 // the coarse grid; the mean pressure is k (mean density - rho0) since the
-// equation of state is linear.
+// equation of state is linear.  kProjLanes lanes per boundary sub-particle,
+// one coarse cell each (the 2x2x2 cells a sub-particle's range spans as a
+// rule; lanes loop over larger spans), over a fixed grid striding over the
+// boundary subs: each lane's chain of dependent loads is one cell long
+// instead of eight.  The group adds sums and counts by shuffles.
+constexpr int kProjLanes = 8;
+constexpr unsigned kProjGrid = 512;
+
+__device__ __forceinline__ float proj_sum(float v) {
+#pragma unroll
+    for (int m = 1; m < kProjLanes; m <<= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 __global__ __launch_bounds__(kB) void k_ts_project(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
                                                    float4* __restrict__ svel, const float4* __restrict__ pos,
                                                    const float4* __restrict__ vel, const int2* __restrict__ range,
                                                    const DevScene* __restrict__ scenes) {
     const uint32_t nsub = cnt->nsub, first = 8u * cnt->nsplit;
-    uint32_t k = first + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsub) return;
+    const uint64_t total = (uint64_t)(nsub > first ? nsub - first : 0u) * kProjLanes;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;   // a multiple of kProjLanes
     const DevScene& S = scenes[0];
-    float4 xs = spos[k];
     const float hs = S.h_sub, hs2 = S.h2_sub;
-    int lx = range_coord(xs.x, -hs, S.ox, S.cell, S.nx), hx = range_coord(xs.x, hs, S.ox, S.cell, S.nx);
-    int ly = range_coord(xs.y, -hs, S.oy, S.cell, S.ny), hy = range_coord(xs.y, hs, S.oy, S.cell, S.ny);
-    int lz = range_coord(xs.z, -hs, S.oz, S.cell, S.nz), hz = range_coord(xs.z, hs, S.oz, S.cell, S.nz);
-    float sum = 0.0f, cntn = 0.0f;
-    for (int kk = lz; kk <= hz; ++kk)
-        for (int j = ly; j <= hy; ++j)
-            for (int i = lx; i <= hx; ++i) {
-                int2 r = range[S.cell_offset + (uint32_t)(S.nx * (S.ny * kk + j) + i)];
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const uint32_t k = first + (uint32_t)(g / kProjLanes);
+        const int lane = (int)(g % kProjLanes);
+        const float4 xs = spos[k];
+        const int lx = range_coord(xs.x, -hs, S.ox, S.cell, S.nx), hx = range_coord(xs.x, hs, S.ox, S.cell, S.nx);
+        const int ly = range_coord(xs.y, -hs, S.oy, S.cell, S.ny), hy = range_coord(xs.y, hs, S.oy, S.cell, S.ny);
+        const int lz = range_coord(xs.z, -hs, S.oz, S.cell, S.nz), hz = range_coord(xs.z, hs, S.oz, S.cell, S.nz);
+        float sum = 0.0f, cntn = 0.0f;
+        if (hx >= lx && hy >= ly && hz >= lz) {   // else a range wrapped by the grid's "% dim": no cells
+            const int nxc = hx - lx + 1, nyc = hy - ly + 1, ncell = nxc * nyc * (hz - lz + 1);
+            for (int c = lane; c < ncell; c += kProjLanes) {
+                const int i = lx + c % nxc, j = ly + (c / nxc) % nyc, kk = lz + c / (nxc * nyc);
+                const int2 r = range[S.cell_offset + (uint32_t)(S.nx * (S.ny * kk + j) + i)];
                 for (int q = r.x; q < r.y; ++q) {
-                    float4 xj = pos[q];
-                    float dx = xs.x - xj.x, dy = xs.y - xj.y, dz = xs.z - xj.z;
+                    const float4 xj = pos[q];
+                    const float dx = xs.x - xj.x, dy = xs.y - xj.y, dz = xs.z - xj.z;
                     if (dx * dx + dy * dy + dz * dz <= hs2) {
                         sum += vel[q].w;
                         cntn += 1.0f;
                     }
                 }
             }
-    svel[k].w = cntn > 0.0f ? sum / cntn : S.rho0;
+        }
+        sum = proj_sum(sum);     // every lane of the group takes part
+        cntn = proj_sum(cntn);
+        if (lane == 0) svel[k].w = cntn > 0.0f ? sum / cntn : S.rho0;
+    }
 }
 
 __global__ __launch_bounds__(kB) void k_ts_hash(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
@@ -725,7 +746,7 @@ void twoscale_enqueue(Engine& e) {
     e.mark(kStageTwoScaleCollect);
     hipLaunchKernelGGL(k_ts_seed, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.split_ids, t.boundary_ids,
                        e.slot_of_oid, e.pos_b, e.vel_b, e.d_scenes, t.spos, t.svel, t.sid);
-    hipLaunchKernelGGL(k_ts_project, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_b,
+    hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_b,
                        e.vel_b, e.range, e.d_scenes);
     e.mark(kStageTwoScaleSeed);
     hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
