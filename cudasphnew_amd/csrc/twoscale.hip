@@ -527,6 +527,8 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
 // as its lane with the most neighbours in the row, not once per candidate
 // that any lane hits.  Rows over 64 candidates take the direct loop for the
 // rest.  The terms are added in the same candidate order as a direct loop.
+// (Four lanes per sub-particle measured slower: 65 us without masks, 58 us
+// with a mask per lane, against 42 us.)
 __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                  const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
                                                  const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
