@@ -695,7 +695,7 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 // arithmetic; the pair loop keeps the gathers of the next three neighbours
 // in flight while it evaluates the current one.
 template <int CAP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_masks(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 48 ? 4 : 3, 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
