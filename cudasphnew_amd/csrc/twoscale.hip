@@ -35,7 +35,7 @@ namespace sph {
 
 struct TSCounts {
     uint32_t nsplit, nboundary, ndefault, nsub, nparents, nsub_prev;
-    uint32_t nmark;   // entries of the split list of this step (k_ts_split); zeroed by k_ts_prev
+    uint32_t nmark;   // entries of the split list of this step (k_ts_split); zeroed by k_ts_force
     uint32_t pad1;
 };
 
@@ -388,23 +388,23 @@ __global__ __launch_bounds__(kB) void k_ts_project(const TSCounts* __restrict__ 
     }
 }
 
+// Sub cell keys of this step's sub-particles, and (clear_prev) the
+// invalidation of the cells of the previous step's sorted sub keys before
+// the sub sort writes this step's ranges.
 __global__ __launch_bounds__(kB) void k_ts_hash(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
                                                 uint32_t* __restrict__ skey, const DevScene* __restrict__ scenes,
-                                                uint32_t* __restrict__ status) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+                                                uint32_t* __restrict__ status, int clear_prev,
+                                                const uint32_t* __restrict__ sskey, int2* __restrict__ srange) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear_prev && k < cnt->nsub_prev) {
+        const uint32_t c = sskey[k];
+        if (k == 0 || sskey[k - 1] != c) srange[c] = make_int2(-1, -1);
+    }
     if (k >= cnt->nsub) return;
     float4 x = spos[k];
     uint32_t flags = 0;
     skey[k] = sub_cell_key(scenes[0], x.x, x.y, x.z, flags);
     if (flags) atomicOr(status, flags);
-}
-
-__global__ __launch_bounds__(kB) void k_ts_cells_clear(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sskey,
-                                                       int2* __restrict__ srange) {
-    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt->nsub_prev) return;
-    uint32_t k = sskey[t];
-    if (t == 0 || sskey[t - 1] != k) srange[k] = make_int2(-1, -1);
 }
 
 // sub cell ranges + gather into sub-sorted order (stable list order = the
@@ -529,14 +529,22 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
 // rest.  The terms are added in the same candidate order as a direct loop.
 // (Four lanes per sub-particle measured slower: 65 us without masks, 58 us
 // with a mask per lane, against 42 us.)
-__global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
+__global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                  const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
                                                  const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
                                                  const float* __restrict__ gridd, float4* __restrict__ spos_out,
                                                  float4* __restrict__ svel_out, float4* __restrict__ sacc,
                                                  uint32_t* __restrict__ status) {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cnt->nsub) return;
+    const uint32_t nsub = cnt->nsub;
+    if (t == 0) {
+        // the last sub kernel of the step: this step's count becomes the
+        // previous one (its cells are cleared next step) and the next
+        // step's split list starts empty (no other thread reads either)
+        cnt->nsub_prev = nsub;
+        cnt->nmark = 0;
+    }
+    if (t >= nsub) return;
     const DevScene& S = scenes[0];
     const float4 xi = spos_s[t];
     const float4 vi = svel_s[t];
@@ -640,13 +648,6 @@ __global__ __launch_bounds__(kB) void k_ts_force(const TSCounts* __restrict__ cn
     svel_out[t] = make_float4(vx, vy, vz, vi.w);
 }
 
-__global__ void k_ts_prev(TSCounts* __restrict__ cnt) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        cnt->nsub_prev = cnt->nsub;
-        cnt->nmark = 0;   // the next step's split list starts empty
-    }
-}
-
 // Sub-particle positions of the last step by sub id, for a renderer:
 // (x, y, z, density) of sorted slot q lands at out[sid[sperm[q]]].
 __global__ __launch_bounds__(kB) void k_ts_export(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
@@ -742,8 +743,6 @@ void twoscale_enqueue(Engine& e) {
     hipLaunchKernelGGL(k_ts_classify, dim3(nblocks(n)), dim3(kB), 0, st, t.state_fin, e.oid_b, n, t.cls);
     e.mark(kStageTwoScaleState);
     hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
-    if (t.sub_cells_valid)
-        hipLaunchKernelGGL(k_ts_cells_clear, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.srange);
     hipLaunchKernelGGL(k_ts_scan, dim3(1), dim3(kB), 0, st, t.blk, t.nblk, t.counts, t.cap, e.status);
     hipLaunchKernelGGL(k_ts_collect, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk, t.counts, t.split_ids,
                        t.boundary_ids, t.default_ids);
@@ -754,7 +753,7 @@ void twoscale_enqueue(Engine& e) {
                        e.vel_b, e.range, e.d_scenes);
     e.mark(kStageTwoScaleSeed);
     hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
-                       e.status);
+                       e.status, t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
     if (t.use_cell_sort) {
         t.csort.sort(st, t.skey, &t.counts->nsub, t.cap, t.sskey, t.sperm_raw, t.srange);
         hipLaunchKernelGGL(k_ts_gather_ranked, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm_raw,
@@ -771,7 +770,6 @@ void twoscale_enqueue(Engine& e) {
                        t.srange, e.d_scenes);
     hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
                        t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
-    hipLaunchKernelGGL(k_ts_prev, dim3(1), dim3(64), 0, st, t.counts);
     e.mark(kStageTwoScaleSubPhysics);
 }
 
