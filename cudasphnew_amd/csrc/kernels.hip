@@ -615,9 +615,13 @@ __device__ __forceinline__ uint64_t density_row(const At& at, int cnt, float4 xi
 // window once with coalesced loads, and the candidate loop reads LDS: one
 // global load per particle and row in place of one per particle and
 // candidate, which leaves the pass bound by its arithmetic instead of the
-// texture data path.  A row whose window does not fit (the wavefront
-// straddles two cell rows or two scenes) reads global memory as before.
-constexpr int kWin = 768;
+// texture data path.  A row whose window does not fit in what is left of
+// the wavefront's kWin entries (the first rows fill it; a wavefront that
+// straddles two cell rows or two scenes needs more) reads global memory.
+// kWin trades rows served from LDS against occupancy (the window is the
+// kernel's LDS): measured at 1M particles, 768 entries (all nine rows, four
+// wavefronts per SIMD) 131 us, 640 124 us, 512 121 us, 384 120 us, 256 135 us.
+constexpr int kWin = 384;
 
 __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                           const int2* __restrict__ range, uint32_t n,
