@@ -693,13 +693,17 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 }
 
 // The hit list holds 16-bit entries (row << 6 | position in the row's run)
-// and the nine run starts per thread, so that four blocks fit the LDS of a
-// CU.  Entries past the capacity go to a spare list row and are evaluated
-// afterwards from the masks, so that the expansion loop carries no pair
-// arithmetic; the pair loop keeps the gathers of the next three neighbours
-// in flight while it evaluates the current one.
+// and the nine run starts per thread: (CAP + 1) * 512 + 9216 bytes of LDS
+// per block, so that five blocks fit the LDS of a CU at CAP = 40 (four at
+// 48, three at 64), and the occupancy hint asks the register allocation for
+// as many wavefronts per SIMD.  Entries past the capacity go to a spare list
+// row and are evaluated afterwards from the masks, so that the expansion
+// loop carries no pair arithmetic; the pair loop keeps the gathers of the
+// next three neighbours in flight while it evaluates the current one.
+// Measured at 1M particles: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4)
+// 132 us, 32 at 6 wavefronts spills and takes 139 us.
 template <int CAP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 48 ? 4 : 3, 8))) void k_force_masks(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
@@ -951,6 +955,7 @@ void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
         switch (lanes) {
         case 32: hipLaunchKernelGGL(k_force_masks<32>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
         case 64: hipLaunchKernelGGL(k_force_masks<64>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
+        case 40: hipLaunchKernelGGL(k_force_masks<40>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
         default: hipLaunchKernelGGL(k_force_masks<48>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
         }
         return;

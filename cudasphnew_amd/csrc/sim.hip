@@ -485,13 +485,13 @@ void Engine::drop_graph() {
 }
 
 static bool lanes_ok(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16; }
-static bool list_ok(int c) { return c == 32 || c == 48 || c == 64; }
+static bool list_ok(int c) { return c == 32 || c == 40 || c == 48 || c == 64; }
 
 sph_status Engine::set_tuning(const sph_tuning& t) {
     if (!lanes_ok(t.density_lanes) || !lanes_ok(t.force_lanes))
         return fail(SPH_INVALID_ARG, "lanes per particle must be 1, 2, 4, 8 or 16");
     if (t.hit_masks != 0 && t.hit_masks != 1) return fail(SPH_INVALID_ARG, "hit_masks must be 0 or 1");
-    if (!list_ok(t.hit_list)) return fail(SPH_INVALID_ARG, "hit_list must be 32, 48 or 64");
+    if (!list_ok(t.hit_list)) return fail(SPH_INVALID_ARG, "hit_list must be 32, 40, 48 or 64");
     if (t.hit_masks && !masks) {
         sph_status st = alloc(&masks, (size_t)9 * N);
         if (st != SPH_OK) return st;
@@ -547,7 +547,7 @@ sph_status Engine::autotune(int reps, sph_tuning* out) {
         return ms;
     };
     static const int cand[] = {1, 2, 4, 8, 16};
-    static const int caps[] = {32, 48, 64};
+    static const int caps[] = {32, 40, 48, 64};
     float best_d = 1e30f, best_f = 1e30f, best_fm = 1e30f;
     sph_tuning pick = tuning;
     for (int L : cand) {

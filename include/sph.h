@@ -308,7 +308,8 @@ sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
  * cooperative variants).  hit_masks = 1: the density pass records the force
  * neighbours of each particle as per-row bit masks (72 bytes per particle of
  * extra device memory) and the force pass runs the pair terms from them
- * through a per-thread hit list of hit_list entries (32, 48 or 64) in LDS;
+ * through a per-thread hit list of hit_list entries (32, 40, 48 or 64) in
+ * LDS;
  * the lanes fields are then unused.  No reference counterpart: the reference
  * fixes one thread per particle and a block size of 256
  * (particle_simulation.cu:1385-1556); these generalise its launch

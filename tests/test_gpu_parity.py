@@ -346,6 +346,7 @@ TUNINGS = [
     dict(density_lanes=16, force_lanes=16),
     dict(density_lanes=4, force_lanes=16),
     dict(hit_masks=1, hit_list=32),
+    dict(hit_masks=1, hit_list=40),
     dict(hit_masks=1, hit_list=48),
     dict(hit_masks=1, hit_list=64),
 ]
@@ -374,7 +375,7 @@ def test_kernel_variants_one_step_parity(dam16k, tuning):
     assert np.max(np.abs(sim.positions() - orc.read(Field.POSITION))) <= 1e-6
 
 
-@pytest.mark.parametrize("tuning", [TUNINGS[1], TUNINGS[2], TUNINGS[4], TUNINGS[6], TUNINGS[8]], ids=_tid)
+@pytest.mark.parametrize("tuning", [TUNINGS[1], TUNINGS[2], TUNINGS[4], TUNINGS[6], TUNINGS[7], TUNINGS[9]], ids=_tid)
 def test_kernel_variants_dense_cluster(dam16k, tuning):
     """Hundreds of neighbours per particle: the cooperative force kernel's
     per-particle hit list overflows (those particles take the direct loop),
@@ -420,7 +421,7 @@ def test_autotune_keeps_parity(dam16k):
     sim.Advance(1)
     t = sim.autotune(reps=2)
     assert t["density_lanes"] in (1, 2, 4, 8, 16) and t["force_lanes"] in (1, 2, 4, 8, 16)
-    assert t["hit_masks"] in (0, 1) and t["hit_list"] in (32, 48, 64)
+    assert t["hit_masks"] in (0, 1) and t["hit_list"] in (32, 40, 48, 64)
     assert sim.get_tuning() == t
     sim.sync()
     orc = O.OracleSim(dam16k, real=0, threads=THREADS)
@@ -435,10 +436,10 @@ def test_autotune_keeps_parity(dam16k):
     with pytest.raises(S.SphError):
         sim.set_tuning(density_lanes=3)
     with pytest.raises(S.SphError):
-        sim.set_tuning(hit_masks=1, hit_list=40)
+        sim.set_tuning(hit_masks=1, hit_list=36)
 
 
-@pytest.mark.parametrize("tuning", [TUNINGS[3], TUNINGS[7]], ids=_tid)
+@pytest.mark.parametrize("tuning", [TUNINGS[3], TUNINGS[7], TUNINGS[8]], ids=_tid)
 def test_batch_tuning_matches_default(tuning):
     scenes = [S.make_scene("batch", index=i, seed=1) for i in (3, 11)]
     a, b = S.Batch(scenes), S.Batch(scenes)
