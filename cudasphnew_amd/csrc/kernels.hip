@@ -701,7 +701,8 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 // loop carries no pair arithmetic; the pair loop keeps the gathers of the
 // next three neighbours in flight while it evaluates the current one.
 // Measured at 1M particles: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4)
-// 132 us, 32 at 6 wavefronts spills and takes 139 us.
+// 132 us, 32 at 6 wavefronts spills and takes 139 us (156 us with two
+// gathers in flight instead of three).
 template <int CAP>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
