@@ -136,6 +136,29 @@ def test_dam1m_one_step_parity_and_sort_properties():
     assert np.array_equal(np.repeat(np.nonzero(occ)[0], (ce - cs)[occ]), sk)
 
 
+def test_dam1m_300_steps_mean_density_and_kinetic_energy():
+    """The full-size workload in the benchmark's configuration (hit-mask pair,
+    40-entry list, graph replay) against the FP64 oracle over 300 steps:
+    mean density within 0.1% and kinetic energy within 1% at every 100."""
+    sc = S.make_scene("dam1m")
+    sim = _sim(sc)
+    sim.set_tuning(hit_masks=1, hit_list=40)
+    sim.use_graph(True)
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    m = sc.params.particle_mass
+    for chunk in range(3):
+        sim.Advance(100)
+        orc.step(100)
+        sim.sync()
+        rho, rho_o = sim.read(Field.DENSITY), orc.read(Field.DENSITY)
+        ke = kinetic_energy(m, sim.read(Field.VELOCITY))
+        ke_o = kinetic_energy(m, orc.read(Field.VELOCITY))
+        print(f"1M step {100 * (chunk + 1)}: mean rho {rho.mean():.5f} vs {rho_o.mean():.5f}, "
+              f"KE {ke:.6g} vs {ke_o:.6g}", flush=True)
+        assert abs(rho.mean() - rho_o.mean()) <= 1e-3 * rho_o.mean()
+        assert abs(ke - ke_o) <= 0.01 * ke_o
+
+
 @pytest.mark.parametrize("n", [1, 63, 2047, 2049, 4097, 10007])
 def test_ragged_sizes(dam16k, n):
     pos = dam16k.positions[:n].copy()
