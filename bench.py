@@ -357,6 +357,9 @@ def main(argv=None) -> int:
     ap.add_argument("--two-scale-profile", type=int, default=0,
                     help="run this many two-scale steps of the 262,144-particle two-scale workload and exit "
                          "(for kernel traces of the sub-particle pass)")
+    ap.add_argument("--batch-profile", type=int, default=0,
+                    help="run this many batched steps of the 64-scene workload (graph replay, autotuned) and exit "
+                         "(for kernel traces of the batched step)")
     ap.add_argument("--train-steps", type=int, default=20000, help="training steps of the 1M scene's network")
     ap.add_argument("--batch-train-steps", type=int, default=2000, help="training steps per batch scene network")
     ap.add_argument("--batch-steps", type=int, default=50, help="timed steps of the batched forms")
@@ -373,6 +376,17 @@ def main(argv=None) -> int:
 
     if a.train_profile:
         return train_profile(S, a)
+    if a.batch_profile:
+        scenes = [S.make_scene("batch", index=i, seed=1) for i in range(64)]
+        b = S.Batch(scenes)
+        b.use_graph(True)
+        b.step(3)
+        b.sync()
+        tun = b.autotune(reps=2)
+        b.step(a.batch_profile)
+        b.sync()
+        log(f"batch profile: {a.batch_profile} steps, 64 scenes, {sum(x.n for x in scenes)} particles, {tun}")
+        return 0
     if a.two_scale_profile:
         sc2 = S.make_scene("two_scale")
         sim2 = S.ParticleSimulation(sc2).Init()
