@@ -220,6 +220,7 @@ SIGNATURES = [
     ("sph_destroy", _st, [_vp]),
     ("sph_step", _st, [_vp, C.c_int32]),
     ("sph_step_two_scale", _st, [_vp, C.c_int32]),
+    ("sph_step_sub_particles", _st, [_vp, C.c_int32]),
     ("sph_sync", _st, [_vp]),
     ("sph_set_params", _st, [_vp, C.POINTER(SphParams)]),
     ("sph_get_params", _st, [_vp, C.POINTER(SphParams)]),

@@ -143,6 +143,8 @@ uint8_t* twoscale_final_state(Engine& e);
 // True once the sub-particle pass has run: its launches are then the same
 // every step (steady state), so the two-scale step can be captured.
 bool twoscale_steady(const Engine& e);
+// One AdvanceSubParticles step of the current sub-particles (enqueued).
+sph_status twoscale_sub_step(Engine& e);
 // Sub-particle positions of the last step by sub id into out (8N float4),
 // and the default / active sub id lists and counts into the view.
 sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view);

@@ -793,6 +793,16 @@ sph_status sph_step(sph_handle* h, int32_t n) { return h ? h->e.step(n, false) :
 
 sph_status sph_step_two_scale(sph_handle* h, int32_t n) { return h ? h->e.step(n, true) : SPH_INVALID_ARG; }
 
+sph_status sph_step_sub_particles(sph_handle* h, int32_t n) {
+    if (!h) return SPH_INVALID_ARG;
+    if (n < 0) return h->e.fail(SPH_INVALID_ARG, "n_steps must be >= 0");
+    for (int i = 0; i < n; ++i) {
+        sph_status st = sph::twoscale_sub_step(h->e);
+        if (st != SPH_OK) return st;
+    }
+    return SPH_OK;
+}
+
 sph_status sph_sync(sph_handle* h) { return h ? h->e.sync() : SPH_INVALID_ARG; }
 
 sph_status sph_set_params(sph_handle* h, const sph_params* p) {

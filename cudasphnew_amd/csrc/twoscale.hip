@@ -648,6 +648,19 @@ __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, con
     svel_out[t] = make_float4(vx, vy, vz, vi.w);
 }
 
+// The integrated sub-particle state of the last step (sorted order) back to
+// list order, as the input of a sub-only step.
+__global__ __launch_bounds__(kB) void k_ts_unsort(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
+                                                  const float4* __restrict__ spos_out,
+                                                  const float4* __restrict__ svel_out, float4* __restrict__ spos,
+                                                  float4* __restrict__ svel) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt->nsub) return;
+    const uint32_t k = sperm[t];
+    spos[k] = spos_out[t];
+    svel[k] = svel_out[t];
+}
+
 // Sub-particle positions of the last step by sub id, for a renderer:
 // (x, y, z, density) of sorted slot q lands at out[sid[sperm[q]]].
 __global__ __launch_bounds__(kB) void k_ts_export(const TSCounts* __restrict__ cnt, const float4* __restrict__ spos,
@@ -732,6 +745,31 @@ void twoscale_destroy(Engine& e) {
     e.ts = nullptr;
 }
 
+// Sub grid and sub physics of the current list-order sub-particles (spos,
+// svel): keys, sort, sorted gather, density, acceleration + integration.
+static void sub_pass(Engine& e, TwoScale& t) {
+    hipStream_t st = e.stream;
+    hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
+                       e.status, t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
+    if (t.use_cell_sort) {
+        t.csort.sort(st, t.skey, &t.counts->nsub, t.cap, t.sskey, t.sperm_raw, t.srange);
+        hipLaunchKernelGGL(k_ts_gather_ranked, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm_raw,
+                           t.srange, t.spos, t.svel, t.sperm, t.spos_s, t.svel_s);
+    } else {
+        t.sorter.sort(st, t.skey, nullptr, t.sskey, t.sperm, &t.counts->nsub, t.sub_bits);
+        hipLaunchKernelGGL(k_ts_cells_gather, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm,
+                           t.spos, t.svel, t.srange, t.spos_s, t.svel_s);
+    }
+    t.sub_cells_valid = true;
+    e.mark(kStageTwoScaleSubSort);
+    const unsigned sub_grid = (unsigned)std::min<uint64_t>(kTsGrid, ((uint64_t)t.cap * kTsLanes + kB - 1) / kB);
+    hipLaunchKernelGGL(k_ts_density, dim3(sub_grid), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
+                       t.srange, e.d_scenes);
+    hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
+                       t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
+    e.mark(kStageTwoScaleSubPhysics);
+}
+
 void twoscale_enqueue(Engine& e) {
     TwoScale& t = *e.ts;
     hipStream_t st = e.stream;
@@ -752,25 +790,25 @@ void twoscale_enqueue(Engine& e) {
     hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_b,
                        e.vel_b, e.range, e.d_scenes);
     e.mark(kStageTwoScaleSeed);
-    hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
-                       e.status, t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
-    if (t.use_cell_sort) {
-        t.csort.sort(st, t.skey, &t.counts->nsub, t.cap, t.sskey, t.sperm_raw, t.srange);
-        hipLaunchKernelGGL(k_ts_gather_ranked, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm_raw,
-                           t.srange, t.spos, t.svel, t.sperm, t.spos_s, t.svel_s);
-    } else {
-        t.sorter.sort(st, t.skey, nullptr, t.sskey, t.sperm, &t.counts->nsub, t.sub_bits);
-        hipLaunchKernelGGL(k_ts_cells_gather, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm,
-                           t.spos, t.svel, t.srange, t.spos_s, t.svel_s);
-    }
-    t.sub_cells_valid = true;
-    e.mark(kStageTwoScaleSubSort);
-    const unsigned sub_grid = (unsigned)std::min<uint64_t>(kTsGrid, ((uint64_t)t.cap * kTsLanes + kB - 1) / kB);
-    hipLaunchKernelGGL(k_ts_density, dim3(sub_grid), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
-                       t.srange, e.d_scenes);
-    hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
-                       t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
-    e.mark(kStageTwoScaleSubPhysics);
+    sub_pass(e, t);
+}
+
+// AdvanceSubParticles (cu:2119-2143): the sub-particles of the last
+// two-scale step advance alone, without reseeding.  Projection takes the
+// coarse state as the last step left it, the integrated positions with the
+// last sort's cell ranges (the same slots) and the last density pass's
+// densities (kept in the positions' w), as the reference pairs its latest
+// positions with its last cell lists.
+sph_status twoscale_sub_step(Engine& e) {
+    if (!twoscale_steady(e)) return e.fail(SPH_INVALID_ARG, "sub-particle step needs a two-scale step first");
+    TwoScale& t = *e.ts;
+    hipStream_t st = e.stream;
+    hipLaunchKernelGGL(k_ts_unsort, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_out, t.svel_out,
+                       t.spos, t.svel);
+    hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_a, e.pos_a,
+                       e.range, e.d_scenes);
+    sub_pass(e, t);
+    return e.check_launch("sub-particle step");
 }
 
 sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view) {

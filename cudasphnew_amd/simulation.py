@@ -105,6 +105,11 @@ class ParticleSimulation:
     def AdvanceTwoScale(self, n: int = 1) -> None:
         check(lib.sph_step_two_scale(self._need(), int(n)), self._h)
 
+    def AdvanceSubParticles(self, n: int = 1) -> None:
+        """AdvanceSubParticles (cu:2119-2143): the sub-particles of the last
+        two-scale step advance alone."""
+        check(lib.sph_step_sub_particles(self._need(), int(n)), self._h)
+
     def sync(self) -> None:
         check(lib.sph_sync(self._need()), self._h)
 

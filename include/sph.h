@@ -193,6 +193,14 @@ sph_status sph_step(sph_handle* h, int32_t n_steps);
  * seeding, projection, sub-grid sort, sub density/acceleration, sub
  * integration and collision.  No host synchronisation inside. */
 sph_status sph_step_two_scale(sph_handle* h, int32_t n_steps);
+/* sph_step_sub_particles: n_steps of AdvanceSubParticles (cu:2119-2143):
+ * the sub-particles of the last two-scale step advance alone (sub grid,
+ * sub density, projection for the boundary subs, sub acceleration,
+ * integration and collision; no reseeding, the coarse particles fixed).
+ * Projection pairs the coarse positions as the last step left them with
+ * that step's cell ranges and densities, as the reference does.
+ * SPH_INVALID_ARG before the first two-scale step. */
+sph_status sph_step_sub_particles(sph_handle* h, int32_t n_steps);
 /* Waits for the handle's stream; reports device-side errors. */
 sph_status sph_sync(sph_handle* h);
 

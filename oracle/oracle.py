@@ -33,6 +33,7 @@ def _load():
                                 C.c_int]),
         "orc_destroy": (None, [vp]),
         "orc_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "orc_step_sub": (C.c_int, [vp, C.c_int]),
         "orc_read": (C.c_long, [vp, C.c_int, vp, C.c_size_t]),
         "orc_counts": (None, [vp, C.POINTER(SphCounts)]),
         "orc_flags": (C.c_uint32, [vp]),
@@ -101,6 +102,10 @@ class OracleSim:
     def step(self, n: int = 1, two_scale: bool = False) -> float:
         lib.orc_step(self._s, n, 1 if two_scale else 0)
         return float(lib.orc_last_step_seconds(self._s))
+
+    def step_sub(self, n: int = 1) -> None:
+        """AdvanceSubParticles: the current sub-particles alone."""
+        lib.orc_step_sub(self._s, n)
 
     def counts(self) -> SphCounts:
         c = SphCounts()

@@ -619,6 +619,17 @@ struct Sim {
         }
     }
 
+    // AdvanceSubParticles (cu:2119-2143): the current sub-particles alone,
+    // no reseeding; projection reads the coarse positions as they are (after
+    // the last integration) with the last sort's cell lists and densities.
+    void step_sub() {
+        project();
+        sub_grid();
+        sub_density();
+        sub_acceleration();
+        integrate_subs();
+    }
+
     void step(bool two_scale) {
         hash_sort_cells();
         density();
@@ -850,6 +861,15 @@ int orc_step(orc_sim* s, int n_steps, int two_scale) {
     double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (s->d) s->d->last_seconds = sec;
     else s->f->last_seconds = sec;
+    return 0;
+}
+
+int orc_step_sub(orc_sim* s, int n_steps) {
+    if (!s) return -1;
+    for (int i = 0; i < n_steps; ++i) {
+        if (s->d) s->d->step_sub();
+        else s->f->step_sub();
+    }
     return 0;
 }
 

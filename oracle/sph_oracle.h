@@ -49,6 +49,8 @@ orc_sim* orc_create_f64(const sph_params* params,
 void orc_destroy(orc_sim* s);
 /* n_steps of the plain step (two_scale = 0) or of AdvanceTwoScale. */
 int orc_step(orc_sim* s, int n_steps, int two_scale);
+/* AdvanceSubParticles (cu:2119-2143): n steps of the current sub-particles. */
+int orc_step_sub(orc_sim* s, int n_steps);
 /* Fields use the sph_field numbering.  Float-valued fields are written as
  * double; integer fields as int32; SPH_FIELD_STATE as uint8.  Returns the
  * number of elements written, or -1 if bytes is too small. */

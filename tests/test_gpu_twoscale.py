@@ -197,6 +197,35 @@ def test_two_scale_graph_replays_match_eager(dam16k):
                 assert np.array_equal(graph.read(f), eager.read(f)), (rep, f)
 
 
+def test_sub_particle_steps(dam16k):
+    """AdvanceSubParticles after a two-scale step: the sub-particles advance
+    alone (no reseeding) against the oracle's restatement, and before any
+    two-scale step the call is refused."""
+    sc = dam16k.with_positions(dam16k.positions)
+    sim = S.ParticleSimulation(sc).Init()
+    with pytest.raises(S.SphError):
+        sim.AdvanceSubParticles(1)
+    sim.AdvanceTwoScale(1)
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1, two_scale=True)
+    orc32 = O.OracleSim(sc, real=1, threads=THREADS)
+    orc32.step(1, two_scale=True)
+    ids = sim.read(Field.SUB_IDS)
+    x0 = sim.read(Field.SUB_POSITION)
+    sim.AdvanceSubParticles(3)
+    orc.step_sub(3)
+    orc32.step_sub(3)
+    sim.sync()
+    c = _compare_two_scale(sim, orc, orc32)
+    assert c.num_sub > 1000
+    # the same sub-particles (no reseeding), moved by the sub steps
+    assert np.array_equal(sim.read(Field.SUB_IDS), ids)
+    x1 = sim.read(Field.SUB_POSITION)
+    assert np.abs(x1 - x0).max() > 0
+    np.testing.assert_allclose(x1, orc.read(Field.SUB_POSITION), rtol=0, atol=1e-5)
+    assert sim.GetNumTimesSteps() == 1
+
+
 def test_two_scale_capacity_exceeded_is_reported(dam16k):
     sc = dam16k.with_positions(dam16k.positions)
     sc.params.max_sub_particles = 800
