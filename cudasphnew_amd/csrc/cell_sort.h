@@ -20,6 +20,7 @@ struct CellSort {
     uint32_t* cnt = nullptr;  // [cells] particles per cell; all zero between sorts
     uint32_t* rank = nullptr; // [capacity] arrival rank of each particle in its cell
     uint32_t* bsum = nullptr; // [blocks] particles per 4096-cell block
+    uint32_t* used = nullptr; // [blocks] nonzero where a key of this sort falls; all zero between sorts
     size_t bytes = 0;
 
     hipError_t init(uint32_t cells, uint32_t capacity);

@@ -1,13 +1,16 @@
 // cell_sort.hip -- counting sort by cell key (see cell_sort.h).
 //
 //   k_cs_count    one atomic per run of equal keys in a wavefront: the
-//                 cells' counts and each particle's arrival rank in its cell
+//                 cells' counts and each particle's arrival rank in its cell;
+//                 flags the 4096-cell blocks that hold a key
 //   k_cs_reduce   totals of 4096-cell blocks (16 counts per thread, read as
-//                 four 16-byte vectors)
+//                 four 16-byte vectors); an unflagged block is empty and
+//                 reads nothing (a fluid filling a tenth of the domain's
+//                 cells touches about a tenth of the blocks)
 //   k_cs_ranges   cell starts = offset of the block (sum of the totals of
 //                 the blocks before it, read by the whole workgroup) + scan
 //                 inside the block; writes range[k] of every occupied cell
-//                 and clears its count
+//                 and clears its count and the block's flag
 //   k_cs_scatter  slot = start of the cell + arrival rank
 // (Block totals accumulated by atomics in k_cs_count were measured slower:
 // neighbouring wavefronts add to the same total and serialise.)
@@ -34,6 +37,7 @@ constexpr int kCsBlockCells = kCsThreads * kCsPerThread;   // 4096
 __global__ __launch_bounds__(kCsThreads) void k_cs_count(const uint32_t* __restrict__ keys, uint32_t n_host,
                                                          const uint32_t* __restrict__ n_dev,
                                                          uint32_t* __restrict__ cnt, uint32_t* __restrict__ rank,
+                                                         uint32_t* __restrict__ used,
                                                          const uint32_t* __restrict__ prev_skey, uint32_t prev_n,
                                                          int2* __restrict__ range) {
     const uint32_t n = n_dev ? *n_dev : n_host;
@@ -57,6 +61,7 @@ __global__ __launch_bounds__(kCsThreads) void k_cs_count(const uint32_t* __restr
         const uint64_t after = lane == 63 ? 0ull : (heads >> (lane + 1));
         const int next = after ? lane + __ffsll((unsigned long long)after) : __popcll(vmask);
         base = atomicAdd(&cnt[k], (uint32_t)(next - lane));
+        used[k / (uint32_t)kCsBlockCells] = 1u;
     }
 
     // the run's head: highest head lane at or below this lane
@@ -109,8 +114,13 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* s_wave, uin
 }
 
 __global__ __launch_bounds__(kCsThreads) void k_cs_reduce(const uint32_t* __restrict__ cnt, uint32_t cells,
+                                                          const uint32_t* __restrict__ used,
                                                           uint32_t* __restrict__ bsum) {
     __shared__ uint32_t s_wave[kCsThreads / 64];
+    if (!used[blockIdx.x]) {   // the same for the whole workgroup
+        if (threadIdx.x == 0) bsum[blockIdx.x] = 0u;
+        return;
+    }
     const uint32_t base = blockIdx.x * (uint32_t)kCsBlockCells + threadIdx.x * (uint32_t)kCsPerThread;
     uint32_t c[kCsPerThread];
     load16(cnt, cells, base, c);
@@ -124,8 +134,12 @@ __global__ __launch_bounds__(kCsThreads) void k_cs_reduce(const uint32_t* __rest
 
 __global__ __launch_bounds__(kCsThreads) void k_cs_ranges(uint32_t* __restrict__ cnt, uint32_t cells,
                                                           const uint32_t* __restrict__ bsum,
+                                                          uint32_t* __restrict__ used,
                                                           int2* __restrict__ range) {
     __shared__ uint32_t s_wave[kCsThreads / 64];
+    // every thread reads the flag before the scans' barriers; thread 0
+    // clears it after them
+    if (!used[blockIdx.x]) return;   // no occupied cell in the block
     const uint32_t base = blockIdx.x * (uint32_t)kCsBlockCells + threadIdx.x * (uint32_t)kCsPerThread;
     uint32_t c[kCsPerThread];
     load16(cnt, cells, base, c);
@@ -147,6 +161,7 @@ __global__ __launch_bounds__(kCsThreads) void k_cs_ranges(uint32_t* __restrict__
         }
         start += c[j];
     }
+    if (threadIdx.x == 0) used[blockIdx.x] = 0u;
 }
 
 __global__ __launch_bounds__(kCsThreads) void k_cs_scatter(const uint32_t* __restrict__ keys, uint32_t n_host,
@@ -178,8 +193,10 @@ hipError_t CellSort::init(uint32_t ncells, uint32_t capacity) {
     if ((e = hipMalloc(&cnt, sizeof(uint32_t) * (padded ? padded : 1))) != hipSuccess) return e;
     if ((e = hipMalloc(&rank, sizeof(uint32_t) * (capacity ? capacity : 1))) != hipSuccess) return e;
     if ((e = hipMalloc(&bsum, sizeof(uint32_t) * (blocks ? blocks : 1))) != hipSuccess) return e;
+    if ((e = hipMalloc(&used, sizeof(uint32_t) * (blocks ? blocks : 1))) != hipSuccess) return e;
     if ((e = hipMemset(cnt, 0, sizeof(uint32_t) * (padded ? padded : 1))) != hipSuccess) return e;
-    bytes = sizeof(uint32_t) * (padded + capacity + blocks);
+    if ((e = hipMemset(used, 0, sizeof(uint32_t) * (blocks ? blocks : 1))) != hipSuccess) return e;
+    bytes = sizeof(uint32_t) * (padded + capacity + 2 * blocks);
     return hipDeviceSynchronize();
 }
 
@@ -187,7 +204,8 @@ void CellSort::release() {
     if (cnt) (void)hipFree(cnt);
     if (rank) (void)hipFree(rank);
     if (bsum) (void)hipFree(bsum);
-    cnt = rank = bsum = nullptr;
+    if (used) (void)hipFree(used);
+    cnt = rank = bsum = used = nullptr;
     bytes = 0;
 }
 
@@ -206,9 +224,9 @@ void CellSort::sort(hipStream_t st, const uint32_t* keys, const uint32_t* n_dev,
 void CellSort::launch(hipStream_t st, const uint32_t* keys, uint32_t n, const uint32_t* n_dev, uint32_t grid_n,
                       uint32_t* skey, uint32_t* perm, int2* range, const uint32_t* prev_skey, uint32_t prev_n) {
     hipLaunchKernelGGL(k_cs_count, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, cnt, rank,
-                       prev_skey, prev_n, range);
-    hipLaunchKernelGGL(k_cs_reduce, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum);
-    hipLaunchKernelGGL(k_cs_ranges, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum, range);
+                       used, prev_skey, prev_n, range);
+    hipLaunchKernelGGL(k_cs_reduce, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, used, bsum);
+    hipLaunchKernelGGL(k_cs_ranges, dim3(blocks), dim3(kCsThreads), 0, st, cnt, cells, bsum, used, range);
     hipLaunchKernelGGL(k_cs_scatter, dim3(grid_for(grid_n)), dim3(kCsThreads), 0, st, keys, n, n_dev, rank, range,
                        skey, perm);
 }
