@@ -418,6 +418,32 @@ def test_two_scale_children_geometry(base):
     assert np.allclose(r, expect[:, None], rtol=1e-7)
 
 
+def test_sub_particle_step_sequencing(base):
+    """AdvanceSubParticles (particle_simulation.cu:2119-2143): the sub-particles
+    of the last two-scale step advance alone, without reseeding, by symplectic
+    Euler with the sub time step (v1 = v0 + dt a1, then x1 = x0 + dt v1, a1
+    being the acceleration this step computed); coarse particles stay put."""
+    sc, pos = two_scale_scene(base, 0)
+    sc.params.collision = 0
+    orc = O.OracleSim(sc, positions=pos)
+    orc.step(1, two_scale=True)
+    n0 = orc.counts().num_sub
+    assert n0 > 0
+    ids = orc.read(Field.SUB_IDS)
+    x0, v0 = orc.read(Field.SUB_POSITION), orc.read(Field.SUB_VELOCITY)
+    coarse = orc.read(Field.POSITION)
+    orc.step_sub(1)
+    assert orc.counts().num_sub == n0
+    assert np.array_equal(orc.read(Field.SUB_IDS), ids)
+    a1 = orc.read(Field.SUB_ACCEL)
+    v1, x1 = orc.read(Field.SUB_VELOCITY), orc.read(Field.SUB_POSITION)
+    dt = float(np.float32(sc.params.time_step_sub))
+    assert np.abs(a1).max() > 0
+    np.testing.assert_allclose(v1, v0 + dt * a1, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(x1, x0 + dt * v1, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(orc.read(Field.POSITION), coarse)
+
+
 def test_two_scale_projection_and_sub_density_brute_force(base):
     sc, pos = two_scale_scene(base, 1)
     orc = O.OracleSim(sc, positions=pos)
