@@ -473,3 +473,48 @@ def test_two_scale_projection_and_sub_density_brute_force(base):
         w = np.maximum(hs * hs - d2, 0.0) ** 3
         assert srho[k] == pytest.approx(ms * poly * w.sum(), rel=1e-12)
     del sx
+
+
+def test_projection_without_coarse_neighbour_falls_back_to_rest_state(base):
+    """project_quantities divides by the number of coarse particles within
+    h_sub (particle_simulation.cu:620-621): 0/0 when there is none.  Right
+    after seeding every boundary sub has its parent within h_sub, but
+    AdvanceSubParticles (cu:2119-2143) moves the subs while the coarse
+    particles stay, so a fast boundary sub leaves every coarse particle
+    behind.  Defined here: rest density and zero pressure."""
+    sc = flat_boundary(base)
+    p = sc.params
+    p.collision = 0
+    p.gravity = 0.0
+    p.sub_placement = 1
+    p.split_lo[:] = [0.295, -1e30, -1e30]
+    p.split_hi[:] = [0.305, 1e30, 1e30]
+    h = float(np.float32(p.compact_support))
+    hs = float(np.float32(p.compact_support_sub))
+    dt = float(np.float32(p.time_step_sub))
+    # one split particle at x = 0.3 and one boundary particle 0.6 h away,
+    # moving away from it fast enough to cross h_sub + h/4 in a few sub steps
+    pos = np.array([[0.30, 0.30, 0.30], [0.30 + 0.6 * h, 0.30, 0.30]])
+    speed = 50.0
+    vel = np.array([[0.0, 0.0, 0.0], [speed, 0.0, 0.0]])
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1, two_scale=True)
+    c = orc.counts()
+    assert c.num_split == 1 and c.num_boundary == 1
+    coarse = orc.read(Field.POSITION)
+    k = 0
+    while True:
+        sx = orc.read(Field.SUB_POSITION).reshape(-1, 3)[8:]      # the boundary subs
+        d2 = ((sx[:, None, :] - coarse[None, :, :]) ** 2).sum(-1)
+        if (d2 > hs * hs * 1.01).all():
+            break
+        orc.step_sub(1)
+        k += 1
+        assert k < 20
+    # the next sub step projects with no coarse neighbour
+    orc.step_sub(1)
+    srho = orc.read(Field.SUB_DENSITY)[8:]
+    spr = orc.read(Field.SUB_PRESSURE)[8:]
+    assert np.all(srho == float(np.float32(p.rest_density)))
+    assert np.all(spr == 0.0)
+    assert k * dt * speed > hs - h / 4
