@@ -57,6 +57,10 @@ class Field(enum.IntEnum):
     SUB_SORTED_KEYS = 19
     SUB_SORTED_IDS = 20
     WALL_SOURCE = 21
+    SUB_CELL_START = 22
+    SUB_CELL_END = 23
+    SUB_INPUT_POSITION = 24
+    SUB_INPUT_VELOCITY = 25
 
 
 class Config(enum.IntEnum):

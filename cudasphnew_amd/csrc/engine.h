@@ -30,8 +30,15 @@ enum Stage {
     kStageTwoScaleSubSort,
     kStageTwoScaleSubPhysics,
     kStageWallMlp,
+    kStageSubStepProject,   // sub-particle step: state back to list order + projection
     kNumStages
 };
+
+// Timing marks that open an interval: a step (counted in the per-step
+// averages) or a sub-particle step (its stages add to the same totals but it
+// is not counted as a step).
+constexpr int kMarkStep = -1;
+constexpr int kMarkSubStep = -2;
 
 const char* stage_name(int s);
 

@@ -565,8 +565,18 @@ sph_status sph_scene_config(int32_t config, int32_t index, uint64_t seed, sph_sc
     case SPH_CONFIG_BATCH: {
         if (index < 0 || index >= 64) return SPH_INVALID_ARG;
         uint64_t base = (uint64_t)index * 16;
+        // log-uniform particle counts, stratified over the 64 scenes: scene
+        // i draws from stratum perm[i] of [0, 1) (a seeded permutation), so
+        // the total stays near its expectation (about 4M) for any seed
+        int perm[64];
+        for (int i = 0; i < 64; ++i) perm[i] = i;
+        for (int i = 63; i > 0; --i) {
+            int j = (int)((double)(i + 1) * unit01(seed, 64 * 16 + (uint64_t)i));
+            std::swap(perm[i], perm[j]);
+        }
         double lnlo = std::log(4096.0), lnhi = std::log(262144.0);
-        double N = std::exp(lnlo + (lnhi - lnlo) * unit01(seed, base + 0));
+        double u = ((double)perm[index] + unit01(seed, base + 0)) / 64.0;
+        double N = std::exp(lnlo + (lnhi - lnlo) * u);
         unsigned ny = (unsigned)std::max(4.0, std::round(std::cbrt(N / 4.0)));
         unsigned nx = 2 * ny, nz = 2 * ny;
         float sp = 0.01f, h = 0.02f;
@@ -575,8 +585,13 @@ sph_status sph_scene_config(int32_t config, int32_t index, uint64_t seed, sph_sc
         float sa = (float)std::sqrt(aspect);
         float margin = 2.0f * (h + sp);
         float lead = h + 3.0f * sp;   // room for the wall band before the block
-        float box[3] = {std::max(2.0f * bx * sa, bx + lead + margin), 4.0f * by,
-                        std::max(2.0f * bz / sa, bz + lead + margin)};
+        // x and z twice the block's, in the drawn aspect ratio, scaled up
+        // where needed so that the free corner region holds an obstacle of
+        // at least 6 h (aspect kept)
+        float wx = 2.0f * bx * sa, wz = 2.0f * bz / sa;
+        float need = lead + 2.0f * margin + 6.0f * h;
+        float grow = std::max(1.0f, std::max((bx + need) / wx, (bz + need) / wz));
+        float box[3] = {wx * grow, 4.0f * by, wz * grow};
         float ext = std::max(box[0], box[2]) + 2.0f * h;
         int bd[3];
         for (int a = 0; a < 3; ++a) bd[a] = std::max(8, (int)std::ceil(127.0f * (box[a] + 2.0f * h) / ext) + 1);

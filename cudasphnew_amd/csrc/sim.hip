@@ -23,7 +23,7 @@ const char* stage_name(int s) {
                                             "cell_ranges",   "reorder",          "density",
                                             "force",         "ts_state",         "ts_collect",
                                             "ts_seed_project", "ts_sub_sort",    "ts_sub_physics",
-                                            "wall_mlp"};
+                                            "wall_mlp",    "sub_step_project"};
     return (s >= 0 && s < kNumStages) ? names[s] : "";
 }
 
@@ -336,7 +336,7 @@ sph_status Engine::resolve_timing() {
     for (auto& r : ev_rec) {
         if (r.first < 0) {
             prev = r.second;
-            ++stage_steps;
+            if (r.first == kMarkStep) ++stage_steps;
             continue;
         }
         float ms = 0.0f;
@@ -353,7 +353,7 @@ sph_status Engine::resolve_timing() {
 
 void Engine::enqueue_step(bool two_scale) {
     StepBuffers b = buffers();
-    mark(-1);
+    mark(kMarkStep);
     if (!keys_valid) launch_hash(stream, b);
     mark(kStageHash);
     // the counting sort clears the previous step's cells itself
@@ -799,6 +799,8 @@ sph_status sph_step_sub_particles(sph_handle* h, int32_t n) {
     for (int i = 0; i < n; ++i) {
         sph_status st = sph::twoscale_sub_step(h->e);
         if (st != SPH_OK) return st;
+        // bound the number of outstanding timing events
+        if (h->e.timing && h->e.ev_rec.size() > 4096 && (st = h->e.resolve_timing()) != SPH_OK) return st;
     }
     return SPH_OK;
 }

@@ -803,10 +803,12 @@ sph_status twoscale_sub_step(Engine& e) {
     if (!twoscale_steady(e)) return e.fail(SPH_INVALID_ARG, "sub-particle step needs a two-scale step first");
     TwoScale& t = *e.ts;
     hipStream_t st = e.stream;
+    e.mark(kMarkSubStep);
     hipLaunchKernelGGL(k_ts_unsort, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_out, t.svel_out,
                        t.spos, t.svel);
     hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_a, e.pos_a,
                        e.range, e.d_scenes);
+    e.mark(kStageSubStepProject);
     sub_pass(e, t);
     return e.check_launch("sub-particle step");
 }
@@ -905,6 +907,32 @@ sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t nbytes) {
         if ((st = cp(sid.data(), t.sid, ns * sizeof(uint32_t))) != SPH_OK || (st = load_perm()) != SPH_OK) return st;
         uint32_t* o = (uint32_t*)dst;
         for (uint32_t i = 0; i < ns; ++i) o[i] = sid[perm[i]];
+        return SPH_OK;
+    }
+    case SPH_FIELD_SUB_CELL_START:
+    case SPH_FIELD_SUB_CELL_END: {
+        if (!need((size_t)t.sub_cells * sizeof(int32_t))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        std::vector<int2> tmp(t.sub_cells);
+        if ((st = cp(tmp.data(), t.srange, (size_t)t.sub_cells * sizeof(int2))) != SPH_OK) return st;
+        int32_t* o = (int32_t*)dst;
+        for (uint32_t q = 0; q < t.sub_cells; ++q) {
+            const int v = f == SPH_FIELD_SUB_CELL_START ? tmp[q].x : tmp[q].y;
+            o[q] = v < 0 ? -1 : v;
+        }
+        return SPH_OK;
+    }
+    case SPH_FIELD_SUB_INPUT_POSITION:
+    case SPH_FIELD_SUB_INPUT_VELOCITY: {
+        if (!need(3 * (size_t)ns * sizeof(float))) return e.fail(SPH_INVALID_ARG, "size mismatch");
+        std::vector<float4> tmp(ns);
+        const float4* src = f == SPH_FIELD_SUB_INPUT_POSITION ? t.spos : t.svel;
+        if ((st = cp(tmp.data(), src, ns * sizeof(float4))) != SPH_OK) return st;
+        float* o = (float*)dst;
+        for (uint32_t k = 0; k < ns; ++k) {
+            o[3 * (size_t)k] = tmp[k].x;
+            o[3 * (size_t)k + 1] = tmp[k].y;
+            o[3 * (size_t)k + 2] = tmp[k].z;
+        }
         return SPH_OK;
     }
     case SPH_FIELD_SPLIT_IDS:

@@ -17,20 +17,23 @@ from ._native import (Field, SphCounts, SphParams, SphRenderView, SphTuning, SPH
 from .scenes import Scene, make_scene
 
 _FLOAT3 = {Field.POSITION, Field.VELOCITY, Field.ACCEL, Field.SUB_POSITION, Field.SUB_VELOCITY,
-           Field.SUB_ACCEL}
+           Field.SUB_ACCEL, Field.SUB_INPUT_POSITION, Field.SUB_INPUT_VELOCITY}
 _FLOAT1 = {Field.DENSITY, Field.PRESSURE, Field.SUB_DENSITY, Field.SUB_PRESSURE}
 _INT = {Field.SORTED_KEYS, Field.SORTED_IDS, Field.CELL_START, Field.CELL_END, Field.SUB_IDS,
         Field.SPLIT_IDS, Field.BOUNDARY_IDS, Field.DEFAULT_IDS, Field.SUB_SORTED_KEYS,
         Field.SUB_SORTED_IDS}
 
 
-def _field_array(field: Field, n: int, ncells: int, counts: SphCounts):
+def _field_array(field: Field, n: int, ncells: int, counts: SphCounts, nsubcells: int = 0):
     nsub = counts.num_sub
     if field in (Field.CELL_START, Field.CELL_END):
         return np.empty(ncells, np.int32)
+    if field in (Field.SUB_CELL_START, Field.SUB_CELL_END):
+        return np.empty(nsubcells, np.int32)
     if field in (Field.STATE, Field.WALL_SOURCE):
         return np.empty(n, np.uint8)
     count = {Field.SUB_POSITION: nsub, Field.SUB_VELOCITY: nsub, Field.SUB_ACCEL: nsub,
+             Field.SUB_INPUT_POSITION: nsub, Field.SUB_INPUT_VELOCITY: nsub,
              Field.SUB_DENSITY: nsub, Field.SUB_PRESSURE: nsub, Field.SUB_IDS: nsub,
              Field.SUB_SORTED_KEYS: nsub, Field.SUB_SORTED_IDS: nsub,
              Field.SPLIT_IDS: counts.num_split, Field.BOUNDARY_IDS: counts.num_boundary,
@@ -138,7 +141,8 @@ class ParticleSimulation:
 
     def read(self, field) -> np.ndarray:
         field = Field(field)
-        out = _field_array(field, self.scene.n, self.num_cells(), self.counts())
+        nsubcells = int(np.prod([self.scene.params.grid_dim_sub[a] for a in range(3)]))
+        out = _field_array(field, self.scene.n, self.num_cells(), self.counts(), nsubcells)
         check(lib.sph_read_field(self._need(), int(field), out.ctypes.data_as(C.c_void_p), out.nbytes),
               self._h)
         return out

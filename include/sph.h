@@ -158,8 +158,21 @@ typedef enum sph_field {
     SPH_FIELD_DEFAULT_IDS = 18, /* int32 [ndefault] (mParticleIdsDefault) */
     SPH_FIELD_SUB_SORTED_KEYS = 19, /* int32 [nsub] (mSubParticleHashs)   */
     SPH_FIELD_SUB_SORTED_IDS = 20,  /* int32 [nsub] sub ids, sorted       */
-    SPH_FIELD_WALL_SOURCE = 21  /* uint8 [n]: boundary source of the last
+    SPH_FIELD_WALL_SOURCE = 21, /* uint8 [n]: boundary source of the last
                                    step (0 none, 1 network, 2 grid) */
+    /* Sub-particle cell tables (mSubParticleCellStartIdsDevPtr /
+     * mSubParticleCellEndIdsDevPtr, particle_simulation.h:269, built by
+     * cu:2371-2389): int32 [sub cells], first / one past the last slot of
+     * each sub cell in SUB_SORTED_* order, -1 = empty. */
+    SPH_FIELD_SUB_CELL_START = 22,
+    SPH_FIELD_SUB_CELL_END = 23,
+    /* float xyz [3*nsub], sub-list order: the sub-particle state the last
+     * sub pass started from -- the positions its sub-grid hash keyed and
+     * the velocities its forces used (the seeded state after a two-scale
+     * step, initialize_sub_particles cu:1023-1090; the state at the start
+     * of the step after a sub-particle step). */
+    SPH_FIELD_SUB_INPUT_POSITION = 24,
+    SPH_FIELD_SUB_INPUT_VELOCITY = 25
 } sph_field;
 
 /* Counts of the two-scale classes (GetNumParticles, GetNumParticlesDefault,
@@ -293,7 +306,9 @@ const char* sph_last_error_message(const sph_handle* h);
  * with HIP events recorded on the handle's stream around every stage
  * (CudaTimer, util.h:193-213, re-expressed); the events are resolved when
  * sph_stage_times is called, so timing adds no host synchronisation to a
- * step.  Stage names: see sph_stage_name. */
+ * step.  Stage names: see sph_stage_name.  Sub-particle steps
+ * (sph_step_sub_particles) add their stage times to the same totals but are
+ * not counted as steps: the averages are per coarse step. */
 #define SPH_MAX_STAGES 16
 sph_status sph_enable_timing(sph_handle* h, int32_t enable);
 sph_status sph_stage_times(sph_handle* h, float* ms, int32_t* count);

@@ -43,6 +43,8 @@ def _load():
         "orc_tex": (None, [C.POINTER(SphBoundary), C.c_int, C.c_uint32, dp, dp]),
         "orc_density_brute": (None, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, dp, dp]),
         "orc_accel_brute": (None, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, dp, dp, dp, dp]),
+        "orc_sub_forces": (C.c_int, [C.POINTER(SphParams), C.POINTER(SphBoundary), C.c_uint32, C.c_uint32, dp, dp,
+                                     dp, C.c_int, dp, dp, dp]),
         "orc_scene_boundary": (C.c_int, [C.POINTER(SceneDesc), dp, dp, dp]),
         "orc_scene_sdf": (None, [C.POINTER(SceneDesc), C.c_uint32, dp, dp]),
         "orc_struct_layout": (None, [C.POINTER(C.c_size_t)]),
@@ -146,6 +148,7 @@ class OracleSim:
         c = self.counts()
         n, nsub = self.n, c.num_sub
         ncells = int(np.prod([self._p.grid_dim[a] for a in range(3)]))
+        nsubcells = int(np.prod([self._p.grid_dim_sub[a] for a in range(3)]))
         shapes = {
             Field.POSITION: ((n, 3), np.float64), Field.VELOCITY: ((n, 3), np.float64),
             Field.ACCEL: ((n, 3), np.float64), Field.DENSITY: ((n,), np.float64),
@@ -158,6 +161,7 @@ class OracleSim:
             Field.SPLIT_IDS: ((c.num_split,), np.int32), Field.BOUNDARY_IDS: ((c.num_boundary,), np.int32),
             Field.DEFAULT_IDS: ((c.num_default,), np.int32), Field.SUB_SORTED_KEYS: ((nsub,), np.int32),
             Field.SUB_SORTED_IDS: ((nsub,), np.int32),
+            Field.SUB_CELL_START: ((nsubcells,), np.int32), Field.SUB_CELL_END: ((nsubcells,), np.int32),
         }
         shape, dt = shapes[field]
         out = np.empty(shape, dt)
@@ -208,6 +212,25 @@ def accel_brute(scene, pos, vel, rho) -> np.ndarray:
     out = np.empty_like(p)
     lib.orc_accel_brute(C.byref(scene.params), C.byref(b), p.shape[0], _dp(p), _dp(v), _dp(r), _dp(out))
     return out
+
+
+def sub_forces(scene, pos, vel, nsplit_subs: int, rho_boundary, threads: int = 8):
+    """FP64 sub-particle density and acceleration on a given state (sub-list
+    order: split children first, then boundary children with the given
+    projected densities).  Returns (rho, acc, colour-gradient norm)."""
+    pos = np.ascontiguousarray(pos, np.float64)
+    vel = np.ascontiguousarray(vel, np.float64)
+    n = pos.shape[0]
+    rin = np.zeros(n, np.float64)
+    rin[nsplit_subs:] = rho_boundary
+    rho = np.empty(n, np.float64)
+    acc = np.empty((n, 3), np.float64)
+    cgn = np.empty(n, np.float64)
+    b = scene.sph_boundary()
+    if lib.orc_sub_forces(C.byref(scene.params), C.byref(b), n, nsplit_subs, _dp(pos), _dp(vel), _dp(rin), threads,
+                          _dp(rho), _dp(acc), _dp(cgn)) < 0:
+        raise ValueError("orc_sub_forces: bad sizes")
+    return rho, acc, cgn
 
 
 def scene_boundary(desc: SceneDesc):

@@ -694,6 +694,8 @@ struct Sim {
         case SPH_FIELD_BOUNDARY_IDS: return put_int(boundary_ids.data(), nboundary);
         case SPH_FIELD_DEFAULT_IDS: return put_int(default_ids.data(), ndefault);
         case SPH_FIELD_SUB_SORTED_KEYS: return put_int(sub_skeys.data(), nsub);
+        case SPH_FIELD_SUB_CELL_START: return put_int(sub_start.data(), sub_start.size());
+        case SPH_FIELD_SUB_CELL_END: return put_int(sub_end.data(), sub_end.size());
         case ORC_FIELD_COLOR_GRAD_NORM: return put_dbl(cgn_out, N);
         case ORC_FIELD_WALL_DISTANCE: return put_dbl(dwall_out, N);
         case ORC_FIELD_SUB_COLOR_GRAD_NORM: return put_dbl(sub_cgn_out, nsub);
@@ -966,6 +968,41 @@ void orc_accel_brute(const sph_params* P, const sph_boundary* b, uint32_t n, con
     for (uint32_t i = 0; i < n; ++i) S.sids[i] = (int32_t)i;
     S.acceleration();
     for (size_t i = 0; i < 3 * (size_t)n; ++i) acc[i] = S.a[i];
+}
+
+int orc_sub_forces(const sph_params* P, const sph_boundary* b, uint32_t nsub, uint32_t nsplit_subs,
+                   const double* pos, const double* vel, const double* rho_in, int threads, double* rho_out,
+                   double* acc, double* cgn) {
+    // The sub-particle physics of one step on a given sub-particle state:
+    // sub grid (compute_sub_particle_hash + sort + compute_cell_start_end,
+    // cu:121-153, 2327-2389), density of the split children (cu:259-313)
+    // and their acceleration (cu:477-568); the boundary children
+    // [nsplit_subs, nsub) carry the densities rho_in (their projection).
+    if (!P || !b || nsub % 8 || nsplit_subs % 8 || nsplit_subs > nsub) return -1;
+    Sim<double> S;
+    S.init(*P, *b, 0, nullptr, nullptr, threads);
+    S.nsub = nsub;
+    S.nsplit = nsplit_subs / 8;
+    S.nboundary = (nsub - nsplit_subs) / 8;
+    S.sx.assign(pos, pos + 3 * (size_t)nsub);
+    S.sv.assign(vel, vel + 3 * (size_t)nsub);
+    S.sa.assign(3 * (size_t)nsub, 0.0);
+    S.srho.assign(nsub, 0.0);
+    S.sp.assign(nsub, 0.0);
+    S.sub_cgn_out.assign(nsub, 1e300);
+    for (uint32_t k = nsplit_subs; k < nsub; ++k) {
+        S.srho[k] = rho_in[k];
+        S.sp[k] = (double)P->gas_stiffness * (rho_in[k] - (double)P->rest_density);
+    }
+    S.sub_grid();
+    S.sub_density();
+    S.sub_acceleration();
+    for (uint32_t k = 0; k < nsub; ++k) {
+        rho_out[k] = S.srho[k];
+        cgn[k] = S.sub_cgn_out[k];
+        for (int c = 0; c < 3; ++c) acc[3 * (size_t)k + c] = S.sa[3 * (size_t)k + c];
+    }
+    return (int)S.flags;
 }
 
 int orc_scene_boundary(const sph_scene_desc* d, double* distance, double* density, double* viscosity) {

@@ -85,6 +85,17 @@ void orc_accel_brute(const sph_params* p, const sph_boundary* b, uint32_t n,
                      const double* pos, const double* vel, const double* rho,
                      double* acc);
 
+/* Sub-particle physics of one step on a given state (FP64): sub grid, the
+ * density of the split children [0, nsplit_subs) and their acceleration;
+ * the boundary children [nsplit_subs, nsub) take the densities rho_in.
+ * Outputs: densities (nsub), accelerations (3 nsub), colour-gradient norms
+ * (nsub; 1e300 where no acceleration pass ran).  Returns the key flags or
+ * -1 on bad sizes. */
+int orc_sub_forces(const sph_params* p, const sph_boundary* b, uint32_t nsub,
+                   uint32_t nsplit_subs, const double* pos, const double* vel,
+                   const double* rho_in, int threads, double* rho_out,
+                   double* acc, double* cgn);
+
 /* ---- boundary grid builder (FP64 restatement of SphInComplexShapes) ---- */
 int orc_scene_boundary(const sph_scene_desc* d, double* distance,
                        double* density, double* viscosity);

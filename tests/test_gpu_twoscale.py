@@ -1,11 +1,17 @@
 """Two-scale pass (AdvanceTwoScale) on the GPU against the oracle.
 
-Classification, lists and sub ids are integer results: bit-exact.  Split
-sub-particle densities within 1e-5 relative like the coarse step; projected
-densities and sub accelerations, which the reference's model makes
-ill-conditioned (count-based mean, cancelling symmetric children), within
-1e-5 / 1e-4 for 99.9% of sub-particles and within twice the deviation of an
-FP32 evaluation of the same algorithm for all.
+Classification, lists and sub ids are integer results: bit-exact.  The sub
+grid (keys, sorted sub ids, sub cell start/end) is bit-exact against the
+oracle's FP32 key arithmetic and stable sort applied to the positions the
+device hashed.  Sub densities and accelerations are compared with the FP64
+oracle evaluated on the device's own sub-particle state (seeded positions
+and velocities, projected densities): density within 1e-5, acceleration
+within 1e-4 of max(|a|, g), as the coarse step.  Compared with the oracle's
+own FP64 seeding, the reference's model is ill-conditioned (the eight
+children of a parent sit on a symmetric cube, so their pressure forces
+nearly cancel and the FP32 rounding of the seeded positions is amplified):
+that comparison keeps a 99.9% bar, within twice the deviation of an FP32
+evaluation of the same algorithm for all.
 """
 import numpy as np
 import pytest
@@ -43,6 +49,91 @@ def _check_sub_accel(sim, orc, orc32, tol=1e-4, margin=2e-5):
         assert e.max() <= max(tol, 2.0 * e32.max())
 
 
+def _sub_grid_params(p):
+    q = S.SphParams.from_buffer_copy(p)
+    q.grid_spacing = p.grid_spacing_sub
+    for a in range(3):
+        q.grid_dim[a] = p.grid_dim_sub[a]
+    return q
+
+
+def _check_sub_grid_exact(sim, params):
+    """compute_sub_particle_hash + sort_by_key + compute_cell_start_end on the
+    sub grid (cu:121-153, 2338-2389): keys from the FP32 expressions, ties in
+    sub-list order, cell start/end -- bit-exact on the positions the device
+    hashed."""
+    c = sim.counts()
+    xh = sim.read(Field.SUB_INPUT_POSITION)
+    ids = sim.read(Field.SUB_IDS)
+    q = _sub_grid_params(params)
+    keys, flags = O.keys(q, xh, real=1)
+    assert flags == 0
+    sk, si, cs, ce = O.sort_cells(q, keys)
+    assert np.array_equal(sim.read(Field.SUB_SORTED_KEYS), sk)
+    assert np.array_equal(sim.read(Field.SUB_SORTED_IDS), ids[si])
+    assert np.array_equal(sim.read(Field.SUB_CELL_START), cs)
+    assert np.array_equal(sim.read(Field.SUB_CELL_END), ce)
+    assert int((cs >= 0).sum()) == len(np.unique(keys)) and c.num_sub == xh.shape[0]
+
+
+def _check_sub_physics_same_inputs(sim, sc, margin=1e-3):
+    """Sub density (split children) within 1e-5 and sub acceleration within
+    1e-4 of max(|a|, g) against the FP64 oracle on the device's own state.
+    Sub-particles whose colour-gradient norm lies within `margin` of
+    normThresh (where FP32 and FP64 may take different branches of the
+    tension switch, cu:551-560) are counted, not compared."""
+    c = sim.counts()
+    nsp = 8 * c.num_split
+    xs = sim.read(Field.SUB_INPUT_POSITION).astype(np.float64)
+    vs = sim.read(Field.SUB_INPUT_VELOCITY).astype(np.float64)
+    rho = sim.read(Field.SUB_DENSITY).astype(np.float64)
+    rho_o, acc_o, cgn = O.sub_forces(sc, xs, vs, nsp, rho[nsp:], threads=THREADS)
+    rel = np.abs(rho[:nsp] - rho_o[:nsp]) / rho_o[:nsp]
+    assert rel.max() <= 1e-5, rel.max()
+    a = sim.read(Field.SUB_ACCEL)[:nsp]
+    ao = acc_o[:nsp]
+    scale = np.maximum(np.linalg.norm(ao, axis=1), G)[:, None]
+    err = (np.abs(a - ao) / scale).max(1)
+    on = np.abs(cgn[:nsp] / float(sc.params.norm_thresh) - 1.0) < margin
+    assert on.sum() <= max(2, 2e-3 * nsp), on.sum()
+    print(f"sub accel on the device's inputs: max {err[~on].max():.3g} ({on.sum()} on the tension switch)")
+    assert err[~on].max() <= 1e-4
+    assert np.all(a[nsp:] == 0) if a.shape[0] > nsp else True
+
+
+def _check_projection_first_step(sim, sc):
+    """project_quantities (cu:570-622) on the device's inputs of the first
+    step: each boundary sub-particle takes the mean density of the coarse
+    particles within h_sub of its seeded position (coarse positions = the
+    initial ones, densities = the device's); within 1e-5, except where a
+    coarse particle sits within 1e-5 h_sub^2 of the cutoff."""
+    from scipy.spatial import cKDTree
+    c = sim.counts()
+    nsp = 8 * c.num_split
+    if c.num_sub == nsp:
+        return
+    xs = sim.read(Field.SUB_INPUT_POSITION).astype(np.float64)[nsp:]
+    rho_c = sim.read(Field.DENSITY).astype(np.float64)
+    xc = sc.positions.astype(np.float64)
+    hs = float(np.float32(sc.params.compact_support_sub))
+    tree = cKDTree(xc)
+    nb = tree.query_ball_point(xs, hs * (1 + 1e-5))
+    got = sim.read(Field.SUB_DENSITY).astype(np.float64)[nsp:]
+    bad = unsure = 0
+    for k, lst in enumerate(nb):
+        lst = np.array(lst, np.int64)
+        d2 = ((xc[lst] - xs[k]) ** 2).sum(1) if lst.size else np.zeros(0)
+        if np.any(np.abs(d2 - hs * hs) < 1e-5 * hs * hs):
+            unsure += 1
+            continue
+        inside = lst[d2 <= hs * hs]
+        exp = rho_c[inside].mean() if inside.size else float(sc.params.rest_density)
+        bad += abs(got[k] - exp) > 1e-5 * exp
+    print(f"projection on the device's inputs: {xs.shape[0]} boundary subs, {unsure} at the cutoff")
+    assert unsure <= max(2, 2e-3 * xs.shape[0])
+    assert bad == 0
+
+
 def _compare_two_scale(sim, orc, orc32):
     c, co = sim.counts(), orc.counts()
     assert (c.num_split, c.num_boundary, c.num_default, c.num_sub) == \
@@ -74,6 +165,8 @@ def _compare_two_scale(sim, orc, orc32):
     p_o = orc.read(Field.SUB_PRESSURE)
     assert np.allclose(p_o, k * (rho_o - rho0), rtol=0, atol=1e-8 * rho0)
     _check_sub_accel(sim, orc, orc32)
+    _check_sub_grid_exact(sim, orc.scene.params)
+    _check_sub_physics_same_inputs(sim, orc.scene)
     assert np.max(np.abs(sim.read(Field.SUB_POSITION) - orc.read(Field.SUB_POSITION))) <= 2e-6
     # sub sort: keys ascending, ties in sub-list order (split parents first,
     # then boundary parents, children in order: the reference's stable sort)
@@ -111,6 +204,7 @@ def test_two_scale_reference_slab_one_step(dam16k, placement):
     orc32.step(1, two_scale=True)
     c = _compare_two_scale(sim, orc, orc32)
     assert c.num_split > 1000 and c.num_boundary > 100
+    _check_projection_first_step(sim, sc)
     # the coarse step is unchanged by the sub pass
     assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
 
@@ -144,6 +238,8 @@ def test_two_scale_state_history_over_steps(dam16k):
         assert np.mean(st == st_o) > 0.999
         c, co = sim.counts(), orc.counts()
         assert abs(int(c.num_split) - int(co.num_split)) <= 0.002 * co.num_split + 2
+        _check_sub_grid_exact(sim, sc.params)
+        _check_sub_physics_same_inputs(sim, sc)
 
 
 def test_two_scale_config_full_size():
@@ -158,6 +254,37 @@ def test_two_scale_config_full_size():
     c = _compare_two_scale(sim, orc, orc32)
     print(f"two-scale config: split {c.num_split}, boundary {c.num_boundary}, subs {c.num_sub}")
     assert c.num_sub > 40000
+    _check_projection_first_step(sim, sc)
+
+
+def test_two_scale_config_500_steps():
+    """AdvanceTwoScale on the two-scale workload (262,144 coarse particles,
+    split region around the sphere) for 500 steps against the FP64 oracle:
+    mean density and kinetic energy of the coarse particles and of the split
+    sub-particles within 1%, the sub-particle count within 1%, and the sub
+    grid bit-exact on the device's positions at the end."""
+    from tests.helpers import kinetic_energy
+    sc = S.make_scene("two_scale")
+    sim = S.ParticleSimulation(sc).Init()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    m, ms = float(sc.params.particle_mass), float(sc.params.sub_particle_mass)
+    for done in range(100, 501, 100):
+        sim.AdvanceTwoScale(100)
+        orc.step(100, two_scale=True)
+        sim.sync()
+        rho, rho_o = sim.read(Field.DENSITY), orc.read(Field.DENSITY)
+        ke, ke_o = kinetic_energy(m, sim.read(Field.VELOCITY)), kinetic_energy(m, orc.read(Field.VELOCITY))
+        c, co = sim.counts(), orc.counts()
+        nsp, nsp_o = 8 * c.num_split, 8 * co.num_split
+        srho, srho_o = sim.read(Field.SUB_DENSITY)[:nsp], orc.read(Field.SUB_DENSITY)[:nsp_o]
+        ske = kinetic_energy(ms, sim.read(Field.SUB_VELOCITY))
+        ske_o = kinetic_energy(ms, orc.read(Field.SUB_VELOCITY))
+        d = {"rho": rho.mean() / rho_o.mean() - 1, "ke": ke / ke_o - 1, "sub_n": c.num_sub / co.num_sub - 1,
+             "sub_rho": srho.mean() / srho_o.mean() - 1, "sub_ke": ske / ske_o - 1}
+        print(f"step {done}: " + ", ".join(f"{k} {v:+.2e}" for k, v in d.items()), flush=True)
+        for k, v in d.items():
+            assert abs(v) <= 0.01, (done, k, v)
+    _check_sub_grid_exact(sim, sc.params)
 
 
 def test_two_scale_then_plain_steps(dam16k):
@@ -224,6 +351,10 @@ def test_sub_particle_steps(dam16k):
     assert np.abs(x1 - x0).max() > 0
     np.testing.assert_allclose(x1, orc.read(Field.SUB_POSITION), rtol=0, atol=1e-5)
     assert sim.GetNumTimesSteps() == 1
+    # the sub grid of the last sub step, keyed on the positions that step
+    # started from, and its physics on that state
+    _check_sub_grid_exact(sim, sc.params)
+    _check_sub_physics_same_inputs(sim, sc)
 
 
 def test_two_scale_capacity_exceeded_is_reported(dam16k):
@@ -236,3 +367,41 @@ def test_two_scale_capacity_exceeded_is_reported(dam16k):
     assert e.value.status == S.Status.CAPACITY_EXCEEDED
     c = sim.counts()
     assert c.num_sub <= 800
+
+
+def test_sub_steps_interleaved_with_graph_replays(dam16k):
+    """AdvanceSubParticles between captured AdvanceTwoScale replays and after
+    plain steps (both sub-particle paths depend on the device-side clearing
+    of the previous sub cells and on slot-consistent coarse state): graph and
+    eager runs agree bit for bit, and the sub-particles follow the oracle."""
+    sc = dam16k.with_positions(dam16k.positions)
+    eager = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
+    graph = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
+    graph.use_graph(True)
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    plan = [("two_scale", 2), ("sub", 2), ("two_scale", 2), ("sub", 1), ("plain", 2), ("sub", 2),
+            ("two_scale", 1), ("sub", 1)]
+    for rep, (kind, n) in enumerate(plan):
+        for sim in (eager, graph):
+            {"two_scale": sim.AdvanceTwoScale, "plain": sim.Advance, "sub": sim.AdvanceSubParticles}[kind](n)
+            sim.sync()
+        if kind == "two_scale":
+            orc.step(n, two_scale=True)
+        elif kind == "plain":
+            orc.step(n)
+        else:
+            orc.step_sub(n)
+        for f in (Field.POSITION, Field.VELOCITY, Field.DENSITY, Field.STATE):
+            assert np.array_equal(graph.read(f), eager.read(f)), (rep, f)
+        c, ce = graph.counts(), eager.counts()
+        assert (c.num_split, c.num_boundary, c.num_sub) == (ce.num_split, ce.num_boundary, ce.num_sub)
+        for f in (Field.SUB_POSITION, Field.SUB_VELOCITY, Field.SUB_DENSITY, Field.SUB_IDS, Field.SUB_SORTED_IDS,
+                  Field.SUB_CELL_START, Field.SUB_CELL_END):
+            assert np.array_equal(graph.read(f), eager.read(f)), (rep, f)
+        co = orc.counts()
+        assert (c.num_split, c.num_boundary, c.num_sub) == (co.num_split, co.num_boundary, co.num_sub), rep
+        assert np.array_equal(graph.read(Field.SUB_IDS), orc.read(Field.SUB_IDS)), rep
+        err = np.abs(graph.read(Field.SUB_POSITION) - orc.read(Field.SUB_POSITION)).max()
+        print(f"{rep} {kind} x{n}: {c.num_sub} subs, position deviation from the oracle {err:.2e}")
+        assert err <= 2e-5, (rep, err)
+        _check_sub_grid_exact(graph, sc.params)

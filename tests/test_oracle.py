@@ -518,3 +518,29 @@ def test_projection_without_coarse_neighbour_falls_back_to_rest_state(base):
     assert np.all(srho == float(np.float32(p.rest_density)))
     assert np.all(spr == 0.0)
     assert k * dt * speed > hs - h / 4
+
+
+def test_sub_forces_on_given_state_matches_the_step(base):
+    """orc_sub_forces (the sub physics on a caller-given state, used to check
+    the device on its own seeded positions) reproduces the two-scale step's
+    sub densities and accelerations when given the step's seeded state."""
+    sc, pos = two_scale_scene(base, 1)
+    sc.params.collision = 0
+    vel = 0.1 * np.random.default_rng(3).standard_normal(pos.shape)
+    orc = O.OracleSim(sc, positions=pos, velocities=vel)
+    orc.step(1, two_scale=True)
+    c = orc.counts()
+    nsp = 8 * c.num_split
+    sid = orc.read(Field.SUB_IDS)
+    par, ch = sid // 8, sid % 8
+    sgn = np.stack([np.where(ch & 4, -1.0, 1.0), np.where(ch & 2, -1.0, 1.0), np.where(ch & 1, -1.0, 1.0)], 1)
+    hq = float(np.float32(sc.params.compact_support)) / 4.0
+    seeded = pos[par] + sgn * hq
+    srho = orc.read(Field.SUB_DENSITY)
+    rho, acc, cgn = O.sub_forces(sc, seeded, vel[par], nsp, srho[nsp:])
+    assert np.array_equal(rho, srho)
+    # the step projects the pressure as the mean of k(rho_j - rho0), the call
+    # forms k(mean rho - rho0): equal up to the last bits
+    a = orc.read(Field.SUB_ACCEL)[:nsp]
+    np.testing.assert_allclose(acc[:nsp], a, rtol=0, atol=1e-12 * np.abs(a).max())
+    assert np.all(acc[nsp:] == 0.0) and np.all(cgn[nsp:] == 1e300)

@@ -198,3 +198,19 @@ def test_batch_configs():
             assert o.kind in (1, 2, 3)
     total = sum(ns)
     assert 2.0e6 < total < 7.0e6
+
+
+def test_batch_generator_and_the_picked_scenes():
+    """The 64-scene workload: log-uniform counts stratified over the scenes
+    (total near the expected ~3.97M), every scene with an obstacle; and the
+    eight scenes the GPU tests run against the oracle
+    (tests/test_gpu_batch.py PICK) are the extremes they are said to be."""
+    all_ = [S.make_scene("batch", index=i, seed=1) for i in range(64)]
+    n = np.array([s.n for s in all_])
+    asp = np.array([(s.desc.box_hi[0] - s.desc.box_lo[0]) / (s.desc.box_hi[2] - s.desc.box_lo[2]) for s in all_])
+    kinds = [sorted(s.desc.obstacles[o].kind for o in range(s.desc.num_obstacles)) for s in all_]
+    assert n[31] == n.min() and n[10] == n.max()
+    assert np.argmin(asp) == 33 and np.argmax(asp) == 17
+    assert kinds[38] == [1] and kinds[47] == [2] and kinds[8] == [3] and kinds[58] == [1, 1]
+    assert all(k for k in kinds)                       # every scene has an obstacle
+    assert 3.5e6 < n.sum() < 4.5e6
