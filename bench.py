@@ -65,15 +65,76 @@ def touched_cells(params, sorted_keys: np.ndarray) -> int:
     return int(np.unique(np.concatenate(seen)).size)
 
 
-# Algorithmic bytes per particle of each single-kernel stage (minimal field
-# sizes, each read or written once per launch): position 12, velocity 12,
-# density 4, acceleration 12, key 4, original id 4.
+# Algorithmic (compulsory) bytes per particle of each single-kernel stage:
+# each minimal unpadded FP32 field read or written once per launch, neighbour
+# re-reads counted as on-chip hits -- position 12, velocity 12, density 4,
+# key 4, id 4.
+#   density: read x, write rho                                   16 B
+#   force + integration + collision + next key: read x, v, rho;
+#     write x, v, key, id                                         60 B
+#   reorder: read key, slot, x, v, id; write x, v, id            64 B
+# plus, for the neighbour passes, 8 B per touched cell-table entry (T_c) and
+# 4 B per distinct boundary texel of each grid they sample.
 STAGE_BYTES_PER_PARTICLE = {
-    "density": 12 + 4,                       # read x, write rho
-    "force": 12 + 12 + 4 + 12 + 12 + 12 + 4 + 4 + 4,   # read x,v,rho,id; write x,v,a,key,id
-    "reorder": 4 + 4 + 12 + 12 + 4 + 12 + 12 + 4,     # read key,slot,x,v,id; write x,v,id
+    "density": 12 + 4,
+    "force": 12 + 12 + 4 + 12 + 12 + 4 + 4,
+    "reorder": 4 + 4 + 12 + 12 + 4 + 12 + 12 + 4,
     "cell_ranges": 4 + 0,
 }
+
+
+def touched_texels(sc, sim) -> dict:
+    """Distinct boundary-grid nodes the trilinear lookups of one step touch
+    (the 8 corners of each lookup): the distance and wall-density lookup at
+    every particle whose cell has a wall term, and, where the particle is in
+    the wall-penalty zone (distance > -r0), the six central-difference
+    lookups of the distance (+-dims/(dims-1) texels) and the viscosity at
+    the centre."""
+    from cudasphnew_amd import Field
+    x = sim.read(Field.POSITION).astype(np.float32)
+    wall = sim.read(Field.WALL_SOURCE) != 0
+    dims = np.array(sc.boundary_dims, np.int64)              # (nx, ny, nz)
+    o = np.array(sc.desc.boundary_origin[:], np.float32)
+    sp = np.float32(sc.desc.boundary_spacing)
+    L = (dims - 1).astype(np.float32) * sp
+    t = (x[wall] - o) / L * dims.astype(np.float32) - np.float32(0.5)
+
+    def corners(tt):
+        tt = np.clip(tt, -1.0, dims.astype(np.float32))
+        f = np.floor(tt).astype(np.int64)
+        out = []
+        for dz in (0, 1):
+            for dy in (0, 1):
+                for dx in (0, 1):
+                    c = np.clip(f + np.array([dx, dy, dz]), 0, dims - 1)
+                    out.append(c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2]))
+        return np.concatenate(out)
+
+    dist = sc.distance.reshape(-1)
+    cen = corners(t)
+    # distance at the particle (trilinear) decides the penalty zone
+    tt = np.clip(t, -1.0, dims.astype(np.float32))
+    f = np.floor(tt)
+    a = tt - f
+    fi = f.astype(np.int64)
+    d = np.zeros(t.shape[0], np.float64)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                c = np.clip(fi + np.array([dx, dy, dz]), 0, dims - 1)
+                w = np.where(dx, a[:, 0], 1 - a[:, 0]) * np.where(dy, a[:, 1], 1 - a[:, 1]) * \
+                    np.where(dz, a[:, 2], 1 - a[:, 2])
+                d += w * dist[c[:, 0] + dims[0] * (c[:, 1] + dims[1] * c[:, 2])]
+    # (a constant -r0 neighbourhood interpolates to -r0 up to rounding here;
+    # the device's a + t (b - a) form keeps it exact)
+    pz = d > -float(sc.desc.rest_distance) * (1.0 - 1e-6)
+    step = dims / (dims - 1)
+    grad = [corners(t[pz] + s * step * e) for e in np.eye(3) for s in (1.0, -1.0)]
+    t_d = np.unique(np.concatenate([cen] + grad)).size
+    t_v = np.unique(corners(t[pz])).size if pz.any() else 0
+    t_rho = np.unique(cen).size
+    return {"T_D": int(t_d), "T_nu": int(t_v), "T_rho": int(t_rho), "wall_particles": int(wall.sum()),
+            "penalty_zone": int(pz.sum())}
 
 
 def copy_probe(nbytes: int) -> None:
@@ -103,7 +164,29 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(scene, seconds: float, threads: int):
+def cpu_topology() -> dict:
+    """Logical CPUs this process may run on and the physical cores they
+    belong to (distinct (physical id, core id) pairs in /proc/cpuinfo)."""
+    allowed = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else list(range(os.cpu_count()))
+    cores, cur = set(), {}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if ":" not in line:
+                if cur.get("processor") in allowed:
+                    cores.add((cur.get("physical id", 0), cur.get("core id", cur["processor"])))
+                cur = {}
+                continue
+            k, v = (t.strip() for t in line.split(":", 1))
+            if k in ("processor", "physical id", "core id"):
+                cur[k] = int(v)
+        if cur.get("processor") in allowed:
+            cores.add((cur.get("physical id", 0), cur.get("core id", cur["processor"])))
+    except (OSError, ValueError):
+        pass
+    return {"logical_cpus": len(allowed), "physical_cores": len(cores) or None}
+
+
+def _cpu_rate(scene, seconds: float, threads: int):
     from oracle import oracle as O
     sim = O.OracleSim(scene, real=1, threads=threads)
     sim.step(1)                                  # warm-up
@@ -111,15 +194,28 @@ def cpu_baseline(scene, seconds: float, threads: int):
     while total < seconds and steps < 200:
         total += sim.step(1)
         steps += 1
+    return scene.n * steps / total, steps, total
+
+
+def cpu_baseline(scene, seconds: float, threads: int):
+    """FP32 C++ transcription of the reference step on the host's cores:
+    all the threads this process may use (the reported value) and one."""
+    topo = cpu_topology()
+    v, steps, total = _cpu_rate(scene, seconds, threads)
+    v1, steps1, total1 = _cpu_rate(scene, seconds / 2, 1)
     return {
-        "value": scene.n * steps / total,
+        "value": v,
         "unit": "particle-updates/s",
         "cores": threads,
-        "cpu": f"{cpu_model()} ({os.cpu_count()} logical CPUs visible)",
+        "cpu": f"{cpu_model()}: {topo['logical_cpus']} logical CPUs on {topo['physical_cores']} physical cores "
+               f"available to this process",
+        "physical_cores": topo["physical_cores"],
         "kind": "port",
         "sample": f"{steps} steps of the same {scene.n}-particle workload after 1 warm-up step, "
                   f"FP32 C++ transcription of the reference step (oracle/, OpenMP, {threads} threads), "
                   f"{total:.1f} s",
+        "single_thread": {"value": v1, "unit": "particle-updates/s", "cores": 1,
+                          "sample": f"{steps1} steps, 1 thread, {total1:.1f} s"},
     }
 
 
@@ -442,7 +538,13 @@ def main(argv=None) -> int:
     tc = touched_cells(sc.params, keys)
     single = {k: v for k, v in stages.items() if k in STAGE_BYTES_PER_PARTICLE and v > 0}
     dom = max(single, key=single.get)
-    algo_bytes = STAGE_BYTES_PER_PARTICLE[dom] * n + (8 * tc if dom in ("density", "force") else 0)
+    tex = touched_texels(sc, sim)
+    algo = {k: STAGE_BYTES_PER_PARTICLE[k] * n for k in single}
+    if "density" in algo:
+        algo["density"] += 8 * tc + 4 * tex["T_rho"]
+    if "force" in algo:
+        algo["force"] += 8 * tc + 4 * (tex["T_D"] + tex["T_nu"])
+    algo_bytes = algo[dom]
     t_dom = single[dom] / 1e3
     achieved = algo_bytes / t_dom / 1e9
     kname = kernel_name(dom, tuning)
@@ -461,6 +563,14 @@ def main(argv=None) -> int:
         "algorithmic_bytes_per_launch": int(algo_bytes),
         "avg_launch_ms": round(single[dom], 5),
         "touched_cells": tc,
+        "touched_texels": tex,
+        # every neighbour pass and streaming pass of the step on the same footing
+        "stages": {k: {"kernel": kernel_name(k, tuning), "avg_launch_ms": round(single[k], 5),
+                       "algorithmic_bytes": int(algo[k]),
+                       "frac": round(algo[k] / (single[k] / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+                       "traffic": profiled_traffic(a.config, kernel_name(k, tuning)),
+                       "profiled_busy": profiled_busy(a.config, kernel_name(k, tuning))}
+                   for k in single},
     }
     forms = None
     if not a.no_forms:

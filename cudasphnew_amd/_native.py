@@ -136,7 +136,7 @@ class SphCounts(C.Structure):
 
 class SphTuning(C.Structure):
     _fields_ = [("density_lanes", C.c_int32), ("force_lanes", C.c_int32),
-                ("hit_masks", C.c_int32), ("hit_list", C.c_int32)]
+                ("hit_masks", C.c_int32), ("hit_list", C.c_int32), ("block", C.c_int32)]
 
 
 class SphSolid(C.Structure):
@@ -172,7 +172,8 @@ class SphMlpConfig(C.Structure):
 
 
 class SphNeuralConfig(C.Structure):
-    _fields_ = [("tolerance", C.c_float), ("band", C.c_float), ("active", C.c_float), ("brick", C.c_int32)]
+    _fields_ = [("tolerance", C.c_float), ("band", C.c_float), ("normal_tolerance", C.c_float),
+                ("brick", C.c_int32)]
 
 
 class SphNeuralReport(C.Structure):
@@ -180,6 +181,7 @@ class SphNeuralReport(C.Structure):
         ("cells_none", C.c_uint64), ("cells_neural", C.c_uint64), ("cells_grid", C.c_uint64),
         ("bricks_checked", C.c_uint32), ("bricks_fallback", C.c_uint32),
         ("max_error", C.c_float), ("mean_error", C.c_float), ("samples_in_band", C.c_uint64),
+        ("mean_normal_error", C.c_float), ("bricks_fallback_normal", C.c_uint32),
     ]
 
 

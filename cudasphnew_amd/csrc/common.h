@@ -54,7 +54,8 @@ enum : uint8_t { kWallNone = 0, kWallNeural = 1, kWallGrid = 2 };
 struct NeuralRefs {
     const uint8_t* mode;    // per global cell key; null: grid everywhere
     const uint32_t* skey;   // sorted keys
-    const float4* rec;      // per sorted slot: (gradient, value) of the learned distance
+    const float4* rec;      // per sorted slot: central difference of the learned distance at
+                            // +-spacing (world units, the grid normal's stencil), 0
     uint8_t* need;          // per sorted slot: learned-source particle in the wall-penalty
                             // zone (written by the density pass, read by the network pass)
     uint32_t* list_count;   // per scene: length of the network pass's particle list,

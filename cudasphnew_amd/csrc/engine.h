@@ -109,7 +109,7 @@ struct Engine {
     NeuralBoundary* nb = nullptr;
 
     // kernel variants of the density / force passes
-    sph_tuning tuning = {1, 1, 0, 48};
+    sph_tuning tuning = {1, 1, 0, 48, 256};
     uint64_t* masks = nullptr;   // [9][N] per-row force-neighbour masks
 
     // render interop buffers, allocated on the first map

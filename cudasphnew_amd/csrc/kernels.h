@@ -48,16 +48,18 @@ void launch_reorder(hipStream_t st, const StepBuffers& b);
 // near_value.
 void launch_wall_modes(hipStream_t st, const DevScene& S, const float4* grid4, uint8_t near_value, uint8_t* mode);
 // lanes: lanes per particle of the neighbour loops (1 = one thread per
-// particle; 2, 4, 8, 16 = wavefront-cooperative variants).  With b.masks set
+// particle; 2, 4, 8, 16 = wavefront-cooperative variants).  block: threads
+// per workgroup of the one-lane and hit-mask variants (128 or 256; the
+// cooperative variants keep 256).  With b.masks set
 // the hit-mask pair is launched instead and the force's lanes value selects
 // its per-thread hit-list capacity (32, 48 or 64).
-void launch_density(hipStream_t st, const StepBuffers& b, int lanes);
+void launch_density(hipStream_t st, const StepBuffers& b, int lanes, int block = 256);
 // Surface flag per original id from the sorted positions and cell ranges of
 // the last step (k_surface).
 void launch_surface(hipStream_t st, const StepBuffers& b, float cm_thresh, float n_thresh, uint8_t* out);
 // out[oid[i]] = (pos[i].xyz, vel[i].w) for i < n: positions by original id.
 void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, const float4* vel, const uint32_t* oid,
                              float4* out);
-void launch_force(hipStream_t st, const StepBuffers& b, int lanes);
+void launch_force(hipStream_t st, const StepBuffers& b, int lanes, int block = 256);
 
 }  // namespace sph

@@ -211,7 +211,7 @@ __device__ __forceinline__ void force_finish(const DevScene& S, uint32_t s, floa
         float4 g = lerp3<float4>(grid4, S, tex_setup(S, tx, ty, tz));
         distWall = -g.x;
         wvisc = g.z;
-        if (wm == kWallNeural && nr.need[s]) r1 = nr.rec[s];   // (grad D, D) of the learned distance texture at x
+        if (wm == kWallNeural && nr.need[s]) r1 = nr.rec[s];   // central-difference gradient of the learned distance at x
     }
     // the network's normal exists where the density pass asked for it
     const bool net_normal = wm == kWallNeural && nr.need[s];
@@ -303,7 +303,8 @@ __device__ __forceinline__ float wall_density(const DevScene& S, const float4* _
 // compute_particle_density_pressure (cu:196-256, 1385-1419), one thread per
 // particle.  Density goes to vel.w; pressure k(rho - rho0) is recomputed
 // where needed.
-__global__ __launch_bounds__(kBlock) void k_density(const float4* __restrict__ pos, float4* __restrict__ vel,
+template <int B>
+__global__ __launch_bounds__(B) void k_density(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                     const int2* __restrict__ range, uint32_t n,
                                                     const DevScene* __restrict__ scenes,
                                                     const uint32_t* __restrict__ begins, int nscenes,
@@ -389,7 +390,8 @@ __global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restric
 // test records the hits in a 64-bit mask, then the pair terms run for the
 // set bits only: roughly one candidate in six is within h, so a fused loop
 // would run the pair body for almost every candidate of a wavefront.
-__global__ __launch_bounds__(kBlock) void k_force(
+template <int B>
+__global__ __launch_bounds__(B) void k_force(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
@@ -546,14 +548,17 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
 }
 
 // Hit-mask pipeline.  The density pass already evaluates the distance of
-// every candidate of the nine row runs; it records, per particle and row,
-// which of the first 64 candidates are force neighbours in a 64-bit mask
-// (masks[row * n + slot], coalesced).  The force pass then skips the
-// distance tests: it expands the masks into a per-thread hit list in LDS and
+// every candidate of the nine row runs; it records, per particle and row, the
+// row's first slot and which of its candidates are force neighbours, packed
+// in one 64-bit word (first slot << 32 | 32-bit mask; masks[row * n + slot],
+// coalesced) when the row holds at most 32 candidates (24 at rest density),
+// and kLongRow otherwise.  The force pass then needs neither the distance
+// tests nor the row runs (no cell-table reads, no neighbour-range
+// divisions): it expands the masks into a per-thread hit list in LDS and
 // runs the pair terms over the list, so a wavefront's trip count is the
 // largest neighbour count of its lanes rather than the sum over rows of the
-// largest per-row count.  Rows longer than 64 candidates (strong
-// compression) are handled by the force pass directly.
+// largest per-row count.  Long rows (strong compression) are handled by the
+// force pass directly.
 //
 // Mask bits: r^2 <= h^2, without the particle's own slot.  A distinct
 // particle at distance 0 keeps its bit; the force pass gives it no
@@ -583,29 +588,27 @@ __device__ __forceinline__ void density_candidate(float xjx, float xjy, float xj
     sum += d * d * d;
 }
 
+constexpr uint64_t kLongRow = ~0ull;   // row of more than 32 candidates (first slots are < 2^31)
+
 // The candidates [0, cnt) of one row run; returns the force-neighbour mask
-// of the first 64 (candidate t at bit t).  at(t) -> (x, y, z) of candidate t.
+// of the first 32 (candidate t at bit t; meaningful when cnt <= 32).
+// at(t) -> (x, y, z) of candidate t.
 template <class At>
-__device__ __forceinline__ uint64_t density_row(const At& at, int cnt, float4 xi, float h2, float& sum) {
-    uint32_t far_lo = 0, far_hi = 0;
-    const int c32 = min(cnt, 32), c64 = min(cnt, 64);
+__device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi, float h2, float& sum) {
+    uint32_t far = 0;
+    const int c32 = min(cnt, 32);
 #pragma unroll 4
     for (int t = 0; t < c32; ++t) {
         const float3 p = at(t);
-        density_candidate(p.x, p.y, p.z, xi, h2, sum, far_lo);
+        density_candidate(p.x, p.y, p.z, xi, h2, sum, far);
     }
-#pragma unroll 4
-    for (int t = 32; t < c64; ++t) {
-        const float3 p = at(t);
-        density_candidate(p.x, p.y, p.z, xi, h2, sum, far_hi);
-    }
-    for (int t = 64; t < cnt; ++t) {
+    for (int t = 32; t < cnt; ++t) {
         const float3 p = at(t);
         float dx = p.x - xi.x, dy = p.y - xi.y, dz = p.z - xi.z;
         float d = fmaxf(fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2))), 0.0f);
         sum += d * d * d;
     }
-    return ((uint64_t)near_bits(far_hi, c64 - 32) << 32) | near_bits(far_lo, c32);
+    return near_bits(far, c32);
 }
 
 // Window of candidate positions per wavefront, in LDS (x, y, z planes).  The
@@ -623,13 +626,14 @@ __device__ __forceinline__ uint64_t density_row(const At& at, int cnt, float4 xi
 // wavefronts per SIMD) 131 us, 640 124 us, 512 121 us, 384 120 us, 256 135 us.
 constexpr int kWin = 384;
 
-__global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                          const int2* __restrict__ range, uint32_t n,
-                                                          const DevScene* __restrict__ scenes,
-                                                          const uint32_t* __restrict__ begins, int nscenes,
-                                                          const float4* __restrict__ grid4,
-                                                          uint64_t* __restrict__ masks, NeuralRefs nr) {
-    __shared__ float s_win[kBlock / 64][3 * kWin];
+template <int B>
+__global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                     const int2* __restrict__ range, uint32_t n,
+                                                     const DevScene* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ begins, int nscenes,
+                                                     const float4* __restrict__ grid4,
+                                                     uint64_t* __restrict__ masks, NeuralRefs nr) {
+    __shared__ float s_win[B / 64][3 * kWin];
     const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if ((s0 & ~63u) >= n) return;   // the whole wavefront lies past the end
     // every lane of a live wavefront takes part in the window copies; lanes
@@ -672,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
         const int b = rb[row], cnt = re[row] - rb[row];
-        uint64_t m;
+        uint32_t m;
         if (inlds[row]) {
             const float* w = win + (b + wofs[row]);
             m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
@@ -683,8 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
         }
         // the particle's own slot is not a force neighbour
         const uint32_t self = s - (uint32_t)b;
-        if (self < (uint32_t)min(cnt, 64)) m &= ~(1ull << self);
-        if (live) masks[(size_t)row * n + s] = m;
+        if (self < (uint32_t)min(cnt, 32)) m &= ~(1u << self);
+        if (live) masks[(size_t)row * n + s] = cnt <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kLongRow;
     }
     if (live) {
         float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
@@ -703,8 +707,8 @@ __global__ __launch_bounds__(kBlock) void k_density_masks(const float4* __restri
 // Measured at 1M particles: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4)
 // 132 us, 32 at 6 wavefronts spills and takes 139 us (156 us with two
 // gathers in flight instead of three).
-template <int CAP>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
+template <int CAP, int B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
@@ -714,8 +718,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 4
     NeuralRefs nr) {
     // one column per thread: no barrier needed, consecutive threads hit
     // consecutive banks
-    __shared__ uint16_t s_list[CAP + 1][kBlock];
-    __shared__ int s_base[9][kBlock];
+    __shared__ uint16_t s_list[CAP + 1][B];
+    __shared__ int s_base[9][B];
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const DevScene& S = scenes[scene_of(begins, nscenes, s)];
@@ -728,30 +732,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 4
     const float h = S.h, h2 = S.h2;
     const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
     const float kr0 = S.stiff * S.rho0;
-    Rows R = neighbour_rows(S, xi);
     PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     int cnt = 0;
     uint32_t long_rows = 0;
     const int tid = threadIdx.x;
 #pragma unroll
     for (int row = 0; row < 9; ++row) {
-        int b, e;
-        row_slots(range, S, R, row / 3, row % 3, xi, b, e);
-        s_base[row][tid] = b;
-        if (e - b > 64) {
+        if (mk[row] == kLongRow) {
             long_rows |= 1u << row;
             continue;
         }
-        uint32_t lo = (uint32_t)mk[row], hi = (uint32_t)(mk[row] >> 32);
+        s_base[row][tid] = (int)(mk[row] >> 32);
+        uint32_t lo = (uint32_t)mk[row];
         while (lo) {
             const int t = __builtin_ctz(lo);
             lo &= lo - 1;
-            s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
-            ++cnt;
-        }
-        while (hi) {
-            const int t = __builtin_ctz(hi) + 32;
-            hi &= hi - 1;
             s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
             ++cnt;
         }
@@ -784,7 +779,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 4
         if (m > 2) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
     }
     if (cnt > CAP || long_rows) {
-        // rare: hit lists longer than the capacity, rows longer than 64
+        // rare: hit lists longer than the capacity, rows longer than 32
+        const Rows R = neighbour_rows(S, xi);
         int k = 0;
 #pragma unroll 1
         for (int row = 0; row < 9; ++row) {
@@ -800,9 +796,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CAP <= 4
                 continue;
             }
             b = s_base[row][tid];
-            uint64_t mm = masks[(size_t)row * n + s];
+            uint32_t mm = (uint32_t)masks[(size_t)row * n + s];   // reloaded: keeps mk out of the pair loop
             while (mm) {
-                const int t = __ffsll((unsigned long long)mm) - 1;
+                const int t = __builtin_ctz(mm);
                 mm &= mm - 1;
                 if (k >= CAP) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
                 ++k;
@@ -913,19 +909,33 @@ void launch_reorder(hipStream_t st, const StepBuffers& b) {
                        b.slot_of_oid);
 }
 
-void launch_density(hipStream_t st, const StepBuffers& b, int lanes) {
-    dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
+void launch_density(hipStream_t st, const StepBuffers& b, int lanes, int block) {
+    const bool small = block == 128;
+    dim3 g(blocks_for(b.n, small ? 128 : kBlock)), t(small ? 128 : kBlock);
     if (b.masks) {
-        hipLaunchKernelGGL(k_density_masks, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
-                           b.nscenes, b.grid4, b.masks, b.nr);
+        if (small)
+            hipLaunchKernelGGL(k_density_masks<128>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
+                               b.nscenes, b.grid4, b.masks, b.nr);
+        else
+            hipLaunchKernelGGL(k_density_masks<kBlock>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes,
+                               b.begins, b.nscenes, b.grid4, b.masks, b.nr);
         return;
     }
+    // the cooperative variants keep 256-thread blocks
+    dim3 gc(blocks_for(b.n, kBlock)), tc(kBlock);
     switch (lanes) {
-    case 2: hipLaunchKernelGGL(k_density_coop<2>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
-    case 4: hipLaunchKernelGGL(k_density_coop<4>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
-    case 8: hipLaunchKernelGGL(k_density_coop<8>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
-    case 16: hipLaunchKernelGGL(k_density_coop<16>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
-    default: hipLaunchKernelGGL(k_density, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 2: hipLaunchKernelGGL(k_density_coop<2>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 4: hipLaunchKernelGGL(k_density_coop<4>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 8: hipLaunchKernelGGL(k_density_coop<8>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    case 16: hipLaunchKernelGGL(k_density_coop<16>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
+    default:
+        if (small)
+            hipLaunchKernelGGL(k_density<128>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
+                               b.nscenes, b.grid4, b.nr);
+        else
+            hipLaunchKernelGGL(k_density<kBlock>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
+                               b.nscenes, b.grid4, b.nr);
+        break;
     }
 }
 
@@ -949,24 +959,37 @@ void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, cons
     hipLaunchKernelGGL(k_export_positions, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, n, pos, vel, oid, out);
 }
 
-void launch_force(hipStream_t st, const StepBuffers& b, int lanes) {
-    dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
+template <int B>
+static void launch_force_masks(hipStream_t st, const StepBuffers& b, int cap) {
+    dim3 g(blocks_for(b.n, B)), t(B);
+    switch (cap) {
+    case 32: hipLaunchKernelGGL((k_force_masks<32, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    case 64: hipLaunchKernelGGL((k_force_masks<64, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    case 40: hipLaunchKernelGGL((k_force_masks<40, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    default: hipLaunchKernelGGL((k_force_masks<48, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    }
+}
+
+void launch_force(hipStream_t st, const StepBuffers& b, int lanes, int block) {
+    const bool small = block == 128;
     if (b.masks) {
         // lanes selects the LDS hit-list capacity of the hit-mask variant
-        switch (lanes) {
-        case 32: hipLaunchKernelGGL(k_force_masks<32>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
-        case 64: hipLaunchKernelGGL(k_force_masks<64>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
-        case 40: hipLaunchKernelGGL(k_force_masks<40>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
-        default: hipLaunchKernelGGL(k_force_masks<48>, g, t, 0, st, SPH_FORCE_ARGS_M); break;
-        }
+        if (small) launch_force_masks<128>(st, b, lanes);
+        else launch_force_masks<kBlock>(st, b, lanes);
         return;
     }
+    dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
     switch (lanes) {
     case 2: hipLaunchKernelGGL(k_force_coop<2>, g, t, 0, st, SPH_FORCE_ARGS); break;
     case 4: hipLaunchKernelGGL(k_force_coop<4>, g, t, 0, st, SPH_FORCE_ARGS); break;
     case 8: hipLaunchKernelGGL(k_force_coop<8>, g, t, 0, st, SPH_FORCE_ARGS); break;
     case 16: hipLaunchKernelGGL(k_force_coop<16>, g, t, 0, st, SPH_FORCE_ARGS); break;
-    default: hipLaunchKernelGGL(k_force, g, t, 0, st, SPH_FORCE_ARGS); break;
+    default:
+        if (small)
+            hipLaunchKernelGGL(k_force<128>, dim3(blocks_for(b.n, 128)), dim3(128), 0, st, SPH_FORCE_ARGS);
+        else
+            hipLaunchKernelGGL(k_force<kBlock>, g, t, 0, st, SPH_FORCE_ARGS);
+        break;
     }
 }
 

@@ -600,6 +600,65 @@ __device__ __forceinline__ void mlp_value_grad(const NetView& net, const float u
     }
 }
 
+// Network output y of the 64-point tile (forward only).  Rows
+// s_j = 16w + 4(lane/16) + j: y[j] (every lane of a row group holds them).
+// Uses two 64 x kLds BF16 LDS tiles; starts with a barrier so that a tile
+// can follow another in the same block.
+__device__ __forceinline__ void mlp_value(const NetView& net, const float u[3], int lane, int w, uint16_t* sA1,
+                                          uint16_t* sA2, float y[4]) {
+    const float* th = net.theta;
+    f32x4 acc[4], dact[4];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = f32x4{0, 0, 0, 0};
+    input_layer(net, u, lane, w, acc, nullptr);
+    activate(th + ob1, acc, dact, lane, w, sA1, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = f32x4{0, 0, 0, 0};
+    rows_x_wt<kH, 4>(sA1, net.w.W2T, lane, w, acc);
+    activate(th + ob2, acc, dact, lane, w, sA2, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = f32x4{0, 0, 0, 0};
+    rows_x_wt<kH, 4>(sA2, net.w.W3T, lane, w, acc);
+    float a3[16];
+    activate(th + ob3, acc, dact, lane, w, nullptr, nullptr, a3);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float part = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) part += a3[4 * c + j] * th[ow4 + 16 * c + (lane & 15)];
+        y[j] = row_sum16(part) + th[ob4];
+    }
+}
+
+// The wall normal the step takes from the network: the central difference
+// of the network's distance at +-spacing along each axis, in world units --
+// the stencil of the grid's normal (particle_simulation.cu:429-439: lookups
+// +-spacing/L apart in normalized coordinates) applied to the network's
+// field, so that the network stands in for exactly the quantity the grid
+// gives.  Six forward passes of the tile (point x of row 16w + lane%16);
+// g[j] for rows s_j as in mlp_value.
+__device__ __forceinline__ void net_cd_normal(const NetView& net, float4 x, float spacing, int lane, int w,
+                                              uint16_t* sA1, uint16_t* sA2, float g[4][3]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) g[j][0] = g[j][1] = g[j][2] = 0.0f;
+    const float inv2s = 0.5f / spacing;
+    for (int a = 0; a < 3; ++a)
+        for (int sgn = 1; sgn >= -1; sgn -= 2) {
+            const float o = (float)sgn * spacing;
+            float u[3], y[4];
+            normalized(net, x.x + (a == 0 ? o : 0.0f), x.y + (a == 1 ? o : 0.0f), x.z + (a == 2 ? o : 0.0f), u);
+            mlp_value(net, u, lane, w, sA1, sA2, y);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                float sc;
+                g[j][a] += (float)sgn * decode(net, y[j], sc) * inv2s;
+            }
+        }
+}
+
 // Distance and its gradient at n points (float4 xyz_): one tile of 64
 // points per block.
 __global__ __launch_bounds__(kThreads) void k_mlp_eval(const float4* __restrict__ pts, uint32_t n, NetView net,
@@ -679,7 +738,7 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
                                                        const DevScene* __restrict__ scenes,
                                                        const NetView* __restrict__ nets,
                                                        float4* __restrict__ rec) {
-    __shared__ __align__(16) uint16_t sA1[kTile * kLds], sA2[kTile * kLds], sdZ[kTile * kLds];
+    __shared__ __align__(16) uint16_t sA1[kTile * kLds], sA2[kTile * kLds];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint2 t = tiles[blockIdx.x];
     const DevScene& S = scenes[t.x];
@@ -689,37 +748,37 @@ __global__ __launch_bounds__(kThreads) void k_wall_mlp(const uint2* __restrict__
     const uint32_t k = k0 + 16 * w + (lane & 15);
     const float4 x = pos[L[k < count ? k : k0]];
     const NetView net = nets[t.x];
-    float u[3], y[4], gu[4][3];
-    normalized(net, x.x, x.y, x.z, u);
-    mlp_value_grad(net, u, lane, w, sA1, sA2, sdZ, y, gu);
+    float g[4][3];
+    net_cd_normal(net, x, S.bspacing, lane, w, sA1, sA2, g);
     if ((lane & 15) < 4) {
         const int j = lane & 15;
         const uint32_t kj = k0 + 16 * w + 4 * (lane >> 4) + j;
-        if (kj < count) {
-            float sc;
-            const float d = decode(net, y[j], sc);
-            rec[L[kj]] = make_float4(sc * gu[j][0] * 2.0f * net.inv_ext[0], sc * gu[j][1] * 2.0f * net.inv_ext[1],
-                                     sc * gu[j][2] * 2.0f * net.inv_ext[2], d);
-        }
+        if (kj < count) rec[L[kj]] = make_float4(g[j][0], g[j][1], g[j][2], 0.0f);
     }
 }
 
 // Validation: per brick of cells, the sum of squared errors of the network
 // (and their count) over the given (x, y, z, distance texture) samples
 // where the label exceeds -band (the field is the constant -r0 farther out
-// on the fluid side).
+// on the fluid side); and, where the label exceeds -r0 (the wall-penalty
+// zone, where the step takes the network's normal), the squared difference
+// between the network's normal and the one it replaces (both central
+// differences at +-spacing, world units), weighted by the penetration.
 __global__ __launch_bounds__(kThreads) void k_validate(const float4* __restrict__ smp, uint32_t n, DevScene S,
                                                        NetView net, float band, int brick, int nbx, int nby,
-                                                       int nbz, float* __restrict__ sq_tight,
-                                                       uint32_t* __restrict__ count, float* __restrict__ sums) {
+                                                       int nbz, const float* __restrict__ gridd,
+                                                       float* __restrict__ sq_tight,
+                                                       uint32_t* __restrict__ count, float* __restrict__ sq_grad,
+                                                       uint32_t* __restrict__ count_grad, float* __restrict__ sums) {
     __shared__ __align__(16) uint16_t sA1[kTile * kLds], sA2[kTile * kLds], sdZ[kTile * kLds];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t base = blockIdx.x * kTile;
     const uint32_t my = base + 16 * w + (lane & 15);
     const float4 x = smp[my < n ? my : n - 1];
-    float u[3], y[4], gu[4][3];
+    float u[3], y[4], gu[4][3], gn[4][3];
     normalized(net, x.x, x.y, x.z, u);
     mlp_value_grad(net, u, lane, w, sA1, sA2, sdZ, y, gu);
+    net_cd_normal(net, x, S.bspacing, lane, w, sA1, sA2, gn);
     if ((lane & 15) < 4) {
         const int j = lane & 15;
         const int row = 4 * (lane >> 4) + j;
@@ -742,20 +801,54 @@ __global__ __launch_bounds__(kThreads) void k_validate(const float4* __restrict_
                 atomicAdd(&sums[0], e);
                 atomicAdd(&sums[1], 1.0f);
             }
+            if (xs.w > -S.r0) {
+                // the step's two normals: the central difference of the
+                // network's distance (as k_wall_mlp records it) and the
+                // grid's normalized-coordinate central difference divided by
+                // L (as the collision uses it), both in world units
+                float tx, ty, tz;
+                texel_coords(S, xs.x, xs.y, xs.z, tx, ty, tz);
+                float3 gg = dist_grad_normalized(gridd, S, tx, ty, tz);
+                gg.x *= S.bscale_x * (1.0f / (float)S.bnx);
+                gg.y *= S.bscale_y * (1.0f / (float)S.bny);
+                gg.z *= S.bscale_z * (1.0f / (float)S.bnz);
+                const float nx = gn[j][0], ny = gn[j][1], nz = gn[j][2];
+                // the normal scales the penalty rho/dt^2 (r0 - distWall) G
+                // (cu:422-445): weighted by the penetration (r0 + D)/r0 the
+                // error is that of the penalty force relative to its value
+                // at a penetration of r0
+                const float wgt = fminf(fmaxf((xs.w + S.r0) / S.r0, 0.0f), 1.0f);
+                const float e2 = wgt * wgt *
+                                 ((nx - gg.x) * (nx - gg.x) + (ny - gg.y) * (ny - gg.y) + (nz - gg.z) * (nz - gg.z));
+                atomicAdd(&sq_grad[b], e2);
+                atomicAdd(&count_grad[b], 1u);
+                atomicAdd(&sums[3], sqrtf(e2));
+                atomicAdd(&sums[4], 1.0f);
+            }
         }
     }
 }
 
-// Cells of bricks whose RMS validation error exceeds tol take the grid.
+// Cells of bricks whose RMS distance error exceeds tol, or whose RMS normal
+// error exceeds tol_n (tol_n > 0), take the grid.
+__device__ __forceinline__ bool brick_fails(uint32_t b, const float* sq_tight, const uint32_t* count,
+                                            const float* sq_grad, const uint32_t* count_grad, float tol,
+                                            float tol_n) {
+    const uint32_t k = count[b], kg = count_grad[b];
+    return (k > 0 && sq_tight[b] > tol * tol * (float)k) ||
+           (tol_n > 0.0f && kg > 0 && sq_grad[b] > tol_n * tol_n * (float)kg);
+}
+
 __global__ void k_apply_bricks(DevScene S, const float* __restrict__ sq_tight, const uint32_t* __restrict__ count,
-                               float tol, int brick, int nbx, int nby, uint8_t* __restrict__ mode) {
+                               const float* __restrict__ sq_grad, const uint32_t* __restrict__ count_grad, float tol,
+                               float tol_n, int brick, int nbx, int nby, uint8_t* __restrict__ mode) {
     const uint32_t ncell = (uint32_t)S.nx * S.ny * S.nz;
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
     const int cx = (int)(c % S.nx), cy = (int)((c / S.nx) % S.ny), cz = (int)(c / ((uint32_t)S.nx * S.ny));
     const uint32_t b = (uint32_t)((cz / brick) * nby + cy / brick) * nbx + cx / brick;
-    const uint32_t k = count[b];
-    if (mode[c] == kWallNeural && k > 0 && sq_tight[b] > tol * tol * (float)k) mode[c] = kWallGrid;
+    if (mode[c] == kWallNeural && brick_fails(b, sq_tight, count, sq_grad, count_grad, tol, tol_n))
+        mode[c] = kWallGrid;
 }
 
 
@@ -909,11 +1002,15 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
     const size_t nbrick = (size_t)nbx * nby * nbz;
     float* et = nullptr;
     uint32_t* cnt = nullptr;
+    float* eg = nullptr;
+    uint32_t* cntg = nullptr;
     float* sums = nullptr;
     float4* dval = nullptr;
     sph_status st = SPH_OK;
     if (hipMalloc(&et, sizeof(float) * nbrick) != hipSuccess ||
-        hipMalloc(&cnt, sizeof(uint32_t) * nbrick) != hipSuccess || hipMalloc(&sums, sizeof(float) * 3) != hipSuccess ||
+        hipMalloc(&cnt, sizeof(uint32_t) * nbrick) != hipSuccess ||
+        hipMalloc(&eg, sizeof(float) * nbrick) != hipSuccess ||
+        hipMalloc(&cntg, sizeof(uint32_t) * nbrick) != hipSuccess || hipMalloc(&sums, sizeof(float) * 5) != hipSuccess ||
         (nval && hipMalloc(&dval, sizeof(float4) * nval) != hipSuccess)) {
         (void)hipGetLastError();
         st = e.fail(SPH_OUT_OF_MEMORY, "validation buffers");
@@ -921,24 +1018,29 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
     if (st == SPH_OK) {
         (void)hipMemsetAsync(et, 0, sizeof(float) * nbrick, e.stream);
         (void)hipMemsetAsync(cnt, 0, sizeof(uint32_t) * nbrick, e.stream);
-        (void)hipMemsetAsync(sums, 0, sizeof(float) * 3, e.stream);
+        (void)hipMemsetAsync(eg, 0, sizeof(float) * nbrick, e.stream);
+        (void)hipMemsetAsync(cntg, 0, sizeof(uint32_t) * nbrick, e.stream);
+        (void)hipMemsetAsync(sums, 0, sizeof(float) * 5, e.stream);
         if (nval) {
             (void)hipMemcpyAsync(dval, val, sizeof(float4) * nval, hipMemcpyHostToDevice, e.stream);
             hipLaunchKernelGGL(nn::k_validate, dim3((unsigned)((nval + nn::kTile - 1) / nn::kTile)), dim3(nn::kThreads),
-                               0, e.stream, dval, (uint32_t)nval, S, nv, cfg.band, br, nbx, nby, nbz, et, cnt, sums);
+                               0, e.stream, dval, (uint32_t)nval, S, nv, cfg.band, br, nbx, nby, nbz, e.gridd, et, cnt,
+                               eg, cntg, sums);
         }
-        hipLaunchKernelGGL(nn::k_apply_bricks, dim3((ncell + 255) / 256), dim3(256), 0, e.stream, S, et, cnt,
-                           cfg.tolerance, br, nbx, nby, mode);
+        hipLaunchKernelGGL(nn::k_apply_bricks, dim3((ncell + 255) / 256), dim3(256), 0, e.stream, S, et, cnt, eg, cntg,
+                           cfg.tolerance, cfg.normal_tolerance, br, nbx, nby, mode);
         if (hipStreamSynchronize(e.stream) != hipSuccess) st = e.fail(SPH_DEVICE_ERROR, "attach kernels failed");
     }
     if (st == SPH_OK && rep) {
         std::vector<uint8_t> hm(ncell);
-        std::vector<float> het(nbrick);
-        std::vector<uint32_t> hc(nbrick);
-        float hs[3] = {0, 0, 0};
+        std::vector<float> het(nbrick), heg(nbrick);
+        std::vector<uint32_t> hc(nbrick), hcg(nbrick);
+        float hs[5] = {0, 0, 0, 0, 0};
         if (hipMemcpy(hm.data(), mode, ncell, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(het.data(), et, sizeof(float) * nbrick, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(hc.data(), cnt, sizeof(uint32_t) * nbrick, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(heg.data(), eg, sizeof(float) * nbrick, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hcg.data(), cntg, sizeof(uint32_t) * nbrick, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(hs, sums, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess) {
             st = e.fail(SPH_DEVICE_ERROR, "report read");
         } else {
@@ -948,16 +1050,21 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
                 else if (v == kWallNeural) ++rep->cells_neural;
                 else ++rep->cells_grid;
             }
+            const float tn = cfg.normal_tolerance;
             for (size_t b = 0; b < nbrick; ++b) {
-                if (hc[b]) ++rep->bricks_checked;
-                if (hc[b] && het[b] > cfg.tolerance * cfg.tolerance * (float)hc[b]) ++rep->bricks_fallback;
+                if (hc[b] || hcg[b]) ++rep->bricks_checked;
+                const bool bad_d = hc[b] && het[b] > cfg.tolerance * cfg.tolerance * (float)hc[b];
+                const bool bad_n = tn > 0.0f && hcg[b] && heg[b] > tn * tn * (float)hcg[b];
+                if (bad_d || bad_n) ++rep->bricks_fallback;
+                if (bad_n && !bad_d) ++rep->bricks_fallback_normal;
             }
             std::memcpy(&rep->max_error, &hs[2], 4);
             rep->mean_error = hs[1] > 0.0f ? hs[0] / hs[1] : 0.0f;
             rep->samples_in_band = (uint64_t)hs[1];
+            rep->mean_normal_error = hs[4] > 0.0f ? hs[3] / hs[4] : 0.0f;
         }
     }
-    void* tmp[] = {et, cnt, sums, dval};
+    void* tmp[] = {et, cnt, eg, cntg, sums, dval};
     for (void* p : tmp)
         if (p) (void)hipFree(p);
     if (st != SPH_OK) return st;

@@ -371,7 +371,7 @@ void Engine::enqueue_step(bool two_scale) {
     }
     launch_reorder(stream, b);
     mark(kStageReorder);
-    launch_density(stream, b, tuning.density_lanes);
+    launch_density(stream, b, tuning.density_lanes, tuning.block);
     mark(kStageDensity);
     if (two_scale) {
         twoscale_enqueue(*this);
@@ -381,7 +381,7 @@ void Engine::enqueue_step(bool two_scale) {
         neural_enqueue(*this);   // normals where the density pass found the wall penalty active
         mark(kStageWallMlp);
     }
-    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes);
+    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes, tuning.block);
     mark(kStageForce);
     last_two_scale = two_scale;
     keys_valid = true;
@@ -487,7 +487,10 @@ void Engine::drop_graph() {
 static bool lanes_ok(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16; }
 static bool list_ok(int c) { return c == 32 || c == 40 || c == 48 || c == 64; }
 
-sph_status Engine::set_tuning(const sph_tuning& t) {
+sph_status Engine::set_tuning(const sph_tuning& t_in) {
+    sph_tuning t = t_in;
+    if (t.block == 0) t.block = 256;
+    if (t.block != 128 && t.block != 256) return fail(SPH_INVALID_ARG, "block must be 128 or 256 (0 = 256)");
     if (!lanes_ok(t.density_lanes) || !lanes_ok(t.force_lanes))
         return fail(SPH_INVALID_ARG, "lanes per particle must be 1, 2, 4, 8 or 16");
     if (t.hit_masks != 0 && t.hit_masks != 1) return fail(SPH_INVALID_ARG, "hit_masks must be 0 or 1");
@@ -498,7 +501,7 @@ sph_status Engine::set_tuning(const sph_tuning& t) {
         bytes += sizeof(uint64_t) * (size_t)9 * N;
     }
     if (t.density_lanes != tuning.density_lanes || t.force_lanes != tuning.force_lanes ||
-        t.hit_masks != tuning.hit_masks || t.hit_list != tuning.hit_list)
+        t.hit_masks != tuning.hit_masks || t.hit_list != tuning.hit_list || t.block != tuning.block)
         drop_graph();
     tuning = t;
     return SPH_OK;
@@ -546,32 +549,58 @@ sph_status Engine::autotune(int reps, sph_tuning* out) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         return ms;
     };
+    // block sizes apply to the one-lane and hit-mask variants (the
+    // cooperative ones keep 256 threads): one block size for the pair
     static const int cand[] = {1, 2, 4, 8, 16};
     static const int caps[] = {32, 40, 48, 64};
-    float best_d = 1e30f, best_f = 1e30f, best_fm = 1e30f;
+    static const int blks[] = {128, 256};
     sph_tuning pick = tuning;
-    for (int L : cand) {
-        float ms = time_it([&] { launch_density(stream, plain, L); });
-        if (ms < best_d) { best_d = ms; pick.density_lanes = L; }
-    }
-    for (int L : cand) {
-        float ms = time_it([&] { launch_force(stream, plain, L); });
-        if (ms < best_f) { best_f = ms; pick.force_lanes = L; }
-    }
-    float t_dm = time_it([&] { launch_density(stream, withm, 1); });
-    for (int c : caps) {
-        float ms = time_it([&] { launch_force(stream, withm, c); });
-        if (ms < best_fm) { best_fm = ms; pick.hit_list = c; }
+    float best_plain = 1e30f, best_masks = 1e30f;
+    sph_tuning pick_plain = tuning, pick_masks = tuning;
+    for (int B : blks) {
+        float best_d = 1e30f, best_f = 1e30f;
+        int ld = 1, lf = 1;
+        for (int L : cand) {
+            if (L > 1 && B != 256) continue;
+            float ms = time_it([&] { launch_density(stream, plain, L, B); });
+            if (ms < best_d) { best_d = ms; ld = L; }
+        }
+        for (int L : cand) {
+            if (L > 1 && B != 256) continue;
+            float ms = time_it([&] { launch_force(stream, plain, L, B); });
+            if (ms < best_f) { best_f = ms; lf = L; }
+        }
+        if (best_d + best_f < best_plain) {
+            best_plain = best_d + best_f;
+            pick_plain.density_lanes = ld;
+            pick_plain.force_lanes = lf;
+            pick_plain.block = B;
+        }
+        float t_dm = time_it([&] { launch_density(stream, withm, 1, B); });
+        for (int c : caps) {
+            float ms = time_it([&] { launch_force(stream, withm, c, B); });
+            if (t_dm + ms < best_masks) {
+                best_masks = t_dm + ms;
+                pick_masks.hit_list = c;
+                pick_masks.block = B;
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (!ok) return fail(SPH_DEVICE_ERROR, "autotune: kernel failed");
-    pick.hit_masks = (t_dm + best_fm < best_d + best_f) ? 1 : 0;
+    if (best_masks < best_plain) {
+        pick = pick_masks;
+        pick.hit_masks = 1;
+    } else {
+        pick = pick_plain;
+        pick.hit_masks = 0;
+    }
     sph_status st = set_tuning(pick);
     if (st != SPH_OK) return st;
     StepBuffers& b = tuning.hit_masks ? withm : plain;
-    launch_density(stream, b, tuning.density_lanes);
-    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes);
+    launch_density(stream, b, tuning.density_lanes, tuning.block);
+    launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes, tuning.block);
     st = check_launch("autotune");
     if (st != SPH_OK) return st;
     if (out) *out = tuning;

@@ -204,48 +204,61 @@ __global__ __launch_bounds__(kB) void k_ts_count(const uint8_t* __restrict__ cls
     if (threadIdx.x < 3) blk[threadIdx.x * nblk + blockIdx.x] = s_c[threadIdx.x];
 }
 
-// ... one block scans the tile counts, writes the totals and the capacity
-// check ...
-__global__ __launch_bounds__(kB) void k_ts_scan(uint32_t* __restrict__ blk, uint32_t nblk, TSCounts* __restrict__ cnt,
-                                                uint32_t cap, uint32_t* __restrict__ status) {
-    __shared__ uint32_t s_w[kB / 64];
+// ... and every element writes its id into its list (collect_ids).  Each
+// block sums the tile counts before its own (its list offsets) and all of
+// them (the totals and the capacity check) itself, so no separate scan pass
+// runs; block 0 publishes the counts.
+__global__ __launch_bounds__(kB) void k_ts_collect(const uint8_t* __restrict__ cls, uint32_t n,
+                                                   const uint32_t* __restrict__ blk, uint32_t nblk, uint32_t cap,
+                                                   TSCounts* __restrict__ cnt, uint32_t* __restrict__ status,
+                                                   uint32_t* __restrict__ split_ids,
+                                                   uint32_t* __restrict__ boundary_ids,
+                                                   uint32_t* __restrict__ default_ids) {
+    __shared__ uint32_t s_w[3][kB / 64];
+    __shared__ uint32_t s_carry[3];
+    __shared__ uint32_t s_red[2][3][kB / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // offsets (tiles before this one) and totals of the three classes
+    uint32_t before[3] = {0, 0, 0}, all3[3] = {0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < nblk; b += kB)
+        for (int t = 0; t < 3; ++t) {
+            const uint32_t v = blk[t * nblk + b];
+            all3[t] += v;
+            if (b < blockIdx.x) before[t] += v;
+        }
+    for (int t = 0; t < 3; ++t)
+        for (int o = 32; o > 0; o >>= 1) {
+            before[t] += __shfl_down(before[t], o, 64);
+            all3[t] += __shfl_down(all3[t], o, 64);
+        }
+    if (lane == 0)
+        for (int t = 0; t < 3; ++t) {
+            s_red[0][t][wave] = before[t];
+            s_red[1][t][wave] = all3[t];
+        }
+    __syncthreads();
     uint32_t tot[3];
     for (int t = 0; t < 3; ++t) {
-        uint32_t carry = 0;
-        uint32_t* row = blk + t * nblk;
-        for (uint32_t b0 = 0; b0 < nblk; b0 += kB) {
-            uint32_t b = b0 + threadIdx.x;
-            uint32_t v = b < nblk ? row[b] : 0u;
-            uint32_t x = v;
-            for (int o = 1; o < 64; o <<= 1) {
-                uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
-            if (lane == 63) s_w[wave] = x;
-            __syncthreads();
-            uint32_t off = 0, all = 0;
-            for (int w = 0; w < kB / 64; ++w) {
-                if (w < wave) off += s_w[w];
-                all += s_w[w];
-            }
-            __syncthreads();
-            if (b < nblk) row[b] = carry + off + x - v;
-            carry += all;
+        uint32_t a = 0, c = 0;
+        for (int w = 0; w < kB / 64; ++w) {
+            a += s_red[0][t][w];
+            c += s_red[1][t][w];
         }
-        tot[t] = carry;
+        if (threadIdx.x == 0) s_carry[t] = a;
+        tot[t] = c;
     }
-    if (threadIdx.x == 0) {
-        uint32_t nsplit = tot[1], nbound = tot[2];
-        uint64_t nsub = 8ull * (nsplit + nbound);
-        if (nsub > cap) {
-            atomicOr(status, kStatusCapacity);
-            // keep whole parents that fit, split parents first
-            uint32_t fit = cap / 8;
-            if (nsplit > fit) { nsplit = fit; nbound = 0; }
-            else if (nsplit + nbound > fit) nbound = fit - nsplit;
-            nsub = 8ull * (nsplit + nbound);
-        }
+    // capacity: keep whole parents that fit, split parents first
+    uint32_t nsplit = tot[1], nbound = tot[2];
+    uint64_t nsub = 8ull * (nsplit + nbound);
+    const bool over = nsub > cap;
+    if (over) {
+        const uint32_t fit = cap / 8;
+        if (nsplit > fit) { nsplit = fit; nbound = 0; }
+        else if (nsplit + nbound > fit) nbound = fit - nsplit;
+        nsub = 8ull * (nsplit + nbound);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (over) atomicOr(status, kStatusCapacity);
         cnt->nsub_prev = cnt->nsub;
         cnt->nsplit = nsplit;
         cnt->nboundary = nbound;
@@ -253,25 +266,12 @@ __global__ __launch_bounds__(kB) void k_ts_scan(uint32_t* __restrict__ blk, uint
         cnt->nparents = nsplit + nbound;
         cnt->nsub = (uint32_t)nsub;
     }
-}
-
-// ... and every element writes its id into its list (collect_ids).
-__global__ __launch_bounds__(kB) void k_ts_collect(const uint8_t* __restrict__ cls, uint32_t n,
-                                                   const uint32_t* __restrict__ blk, uint32_t nblk,
-                                                   const TSCounts* __restrict__ cnt, uint32_t* __restrict__ split_ids,
-                                                   uint32_t* __restrict__ boundary_ids,
-                                                   uint32_t* __restrict__ default_ids) {
-    __shared__ uint32_t s_w[3][kB / 64];
-    __shared__ uint32_t s_carry[3];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (threadIdx.x < 3) s_carry[threadIdx.x] = blk[threadIdx.x * nblk + blockIdx.x];
     __syncthreads();
-    const uint32_t nsplit = cnt->nsplit, nbound = cnt->nboundary;
     uint32_t base = blockIdx.x * kCompactTile;
     for (int r = 0; r < kCompactItems; ++r) {
         uint32_t i = base + r * kB + threadIdx.x;
         int c = i < n ? cls[i] : -1;
-        uint32_t rank[3], tot[3];
+        uint32_t rank[3], tt[3];
         for (int t = 0; t < 3; ++t) {
             uint64_t m = __ballot(c == t);
             uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -286,13 +286,13 @@ __global__ __launch_bounds__(kB) void k_ts_collect(const uint8_t* __restrict__ c
                 all += s_w[t][w];
             }
             rank[t] += off + s_carry[t];
-            tot[t] = all;
+            tt[t] = all;
         }
         if (c == 1 && rank[1] < nsplit) split_ids[rank[1]] = i;
         if (c == 2 && rank[2] < nbound) boundary_ids[rank[2]] = i;
         if (c == 0) default_ids[rank[0]] = i;
         __syncthreads();
-        if (threadIdx.x < 3) s_carry[threadIdx.x] += tot[threadIdx.x];
+        if (threadIdx.x < 3) s_carry[threadIdx.x] += tt[threadIdx.x];
         __syncthreads();
     }
 }
@@ -300,14 +300,25 @@ __global__ __launch_bounds__(kB) void k_ts_collect(const uint8_t* __restrict__ c
 // initialize_sub_particles for every split and boundary parent (cu:1023-
 // 1090): 8 children, directions (+-1,+-1,+-1) in the order of the reference
 // table (x sign from bit 2, y from bit 1, z from bit 0 of the child index).
+// The sub cell keys of the seeded positions (compute_sub_particle_hash,
+// cu:121-153) are computed here too, and (clear_prev) the cells of the
+// previous step's sorted sub keys are invalidated before the sub sort
+// writes this step's ranges (what k_ts_hash does for a sub-particle step).
 __global__ __launch_bounds__(kB) void k_ts_seed(const TSCounts* __restrict__ cnt,
                                                 const uint32_t* __restrict__ split_ids,
                                                 const uint32_t* __restrict__ boundary_ids,
                                                 const uint32_t* __restrict__ slot_of_oid,
                                                 const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                 const DevScene* __restrict__ scenes, float4* __restrict__ spos,
-                                                float4* __restrict__ svel, uint32_t* __restrict__ sid) {
+                                                float4* __restrict__ svel, uint32_t* __restrict__ sid,
+                                                uint32_t* __restrict__ skey, uint32_t* __restrict__ status,
+                                                int clear_prev, const uint32_t* __restrict__ sskey,
+                                                int2* __restrict__ srange) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (clear_prev && k < cnt->nsub_prev) {
+        const uint32_t pc = sskey[k];
+        if (k == 0 || sskey[k - 1] != pc) srange[pc] = make_int2(-1, -1);
+    }
     const uint32_t nsub = cnt->nsub, nsplit = cnt->nsplit;
     if (k >= nsub) return;
     const DevScene& S = scenes[0];
@@ -328,9 +339,13 @@ __global__ __launch_bounds__(kB) void k_ts_seed(const TSCounts* __restrict__ cnt
     }
     float off = radius * dirf;
     float ox = (c & 4) ? -off : off, oy = (c & 2) ? -off : off, oz = (c & 1) ? -off : off;
-    spos[k] = make_float4(xp.x + ox, xp.y + oy, xp.z + oz, 0.0f);
+    const float4 xs = make_float4(xp.x + ox, xp.y + oy, xp.z + oz, 0.0f);
+    spos[k] = xs;
     svel[k] = make_float4(vp.x, vp.y, vp.z, 0.0f);
     sid[k] = 8u * id + c;
+    uint32_t flags = 0;
+    skey[k] = sub_cell_key(S, xs.x, xs.y, xs.z, flags);
+    if (flags) atomicOr(status, flags);
 }
 
 // project_quantities (cu:570-622, 1328-1381): boundary sub-particles take the
@@ -747,10 +762,11 @@ void twoscale_destroy(Engine& e) {
 
 // Sub grid and sub physics of the current list-order sub-particles (spos,
 // svel): keys, sort, sorted gather, density, acceleration + integration.
-static void sub_pass(Engine& e, TwoScale& t) {
+static void sub_pass(Engine& e, TwoScale& t, bool keys_ready) {
     hipStream_t st = e.stream;
-    hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
-                       e.status, t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
+    if (!keys_ready)
+        hipLaunchKernelGGL(k_ts_hash, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.spos, t.skey, e.d_scenes,
+                           e.status, t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
     if (t.use_cell_sort) {
         t.csort.sort(st, t.skey, &t.counts->nsub, t.cap, t.sskey, t.sperm_raw, t.srange);
         hipLaunchKernelGGL(k_ts_gather_ranked, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sskey, t.sperm_raw,
@@ -781,16 +797,16 @@ void twoscale_enqueue(Engine& e) {
     hipLaunchKernelGGL(k_ts_classify, dim3(nblocks(n)), dim3(kB), 0, st, t.state_fin, e.oid_b, n, t.cls);
     e.mark(kStageTwoScaleState);
     hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
-    hipLaunchKernelGGL(k_ts_scan, dim3(1), dim3(kB), 0, st, t.blk, t.nblk, t.counts, t.cap, e.status);
-    hipLaunchKernelGGL(k_ts_collect, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk, t.counts, t.split_ids,
-                       t.boundary_ids, t.default_ids);
+    hipLaunchKernelGGL(k_ts_collect, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk, t.cap, t.counts,
+                       e.status, t.split_ids, t.boundary_ids, t.default_ids);
     e.mark(kStageTwoScaleCollect);
     hipLaunchKernelGGL(k_ts_seed, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.split_ids, t.boundary_ids,
-                       e.slot_of_oid, e.pos_b, e.vel_b, e.d_scenes, t.spos, t.svel, t.sid);
+                       e.slot_of_oid, e.pos_b, e.vel_b, e.d_scenes, t.spos, t.svel, t.sid, t.skey, e.status,
+                       t.sub_cells_valid ? 1 : 0, t.sskey, t.srange);
     hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_b,
                        e.vel_b, e.range, e.d_scenes);
     e.mark(kStageTwoScaleSeed);
-    sub_pass(e, t);
+    sub_pass(e, t, true);
 }
 
 // AdvanceSubParticles (cu:2119-2143): the sub-particles of the last
@@ -809,7 +825,7 @@ sph_status twoscale_sub_step(Engine& e) {
     hipLaunchKernelGGL(k_ts_project, dim3(kProjGrid), dim3(kB), 0, st, t.counts, t.spos, t.svel, e.pos_a, e.pos_a,
                        e.range, e.d_scenes);
     e.mark(kStageSubStepProject);
-    sub_pass(e, t);
+    sub_pass(e, t, false);
     return e.check_launch("sub-particle step");
 }
 

@@ -145,14 +145,17 @@ class NeuralSDF:
         return out
 
 
-def default_neural_config(desc: SceneDesc, tolerance_h: float = 0.05, band_h: float = 2.0, brick: int = 4):
-    """Validation bar: RMS error tolerance_h * h per brick over samples whose
-    label exceeds -band_h * h."""
+def default_neural_config(desc: SceneDesc, tolerance_h: float = 0.05, band_h: float = 2.0, brick: int = 4,
+                          normal_tolerance: float = 0.05):
+    """Validation bars per brick: RMS distance error tolerance_h * h over
+    samples whose label exceeds -band_h * h, and RMS wall-normal error
+    normal_tolerance (|grad d_network - grad d_grid|, world units) over
+    samples in the wall-penalty zone."""
     h = float(desc.h)
     cfg = SphNeuralConfig()
     cfg.tolerance = tolerance_h * h
     cfg.band = band_h * h
-    cfg.active = 0.0
+    cfg.normal_tolerance = float(normal_tolerance)
     cfg.brick = int(brick)
     return cfg
 

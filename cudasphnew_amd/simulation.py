@@ -251,8 +251,8 @@ class ParticleSimulation:
 
     # -- kernel tuning (variants of the density / force passes) ---------------
     def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
-                   hit_list: int = 48) -> None:
-        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list))
+                   hit_list: int = 48, block: int = 256) -> None:
+        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list), int(block))
         check(lib.sph_set_tuning(self._need(), C.byref(t)), self._h)
 
     def get_tuning(self) -> dict:
@@ -335,8 +335,8 @@ class Batch:
         return attach(self, net, self.scenes[scene].desc, scene, val, cfg)
 
     def set_tuning(self, density_lanes: int = 1, force_lanes: int = 1, hit_masks: int = 0,
-                   hit_list: int = 48) -> None:
-        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list))
+                   hit_list: int = 48, block: int = 256) -> None:
+        t = SphTuning(int(density_lanes), int(force_lanes), int(hit_masks), int(hit_list), int(block))
         check(lib.sph_batch_set_tuning(self._b, C.byref(t)), batch=self._b)
 
     def get_tuning(self) -> dict:

@@ -335,13 +335,16 @@ sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
  * LDS;
  * the lanes fields are then unused.  No reference counterpart: the reference
  * fixes one thread per particle and a block size of 256
- * (particle_simulation.cu:1385-1556); these generalise its launch
+ * (particle_simulation.cu:1385-1556, 2943-2957); these generalise its launch
  * configuration. */
 typedef struct sph_tuning {
     int32_t density_lanes;
     int32_t force_lanes;
     int32_t hit_masks;
     int32_t hit_list;
+    /* threads per workgroup of the one-lane and hit-mask variants of both
+     * passes: 128 or 256 (0 = 256); the cooperative variants keep 256 */
+    int32_t block;
 } sph_tuning;
 sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t);
 sph_status sph_get_tuning(const sph_handle* h, sph_tuning* out);
@@ -606,11 +609,13 @@ const char* sph_mlp_last_error_message(const sph_mlp* m);
  *              far-field values (distance -r0, no wall density or
  *              viscosity): no wall term, no lookup (exactly what the grid
  *              gives there);
- *   network -- per step, the network's gradient at each particle gives
- *              the normal of the wall penalty and of the collision response
- *              (replacing the grid's central differences, 12 trilinear
- *              lookups); the network is trained on the distance texture
- *              (sph_scene_grid_samples).  The distance itself, which
+ *   network -- per step, the central difference of the network's
+ *              distance at +-spacing along each axis (the stencil of the
+ *              grid's normal, cu:429-439, applied to the network's field:
+ *              six forward passes on MFMA) gives the normal of the wall
+ *              penalty and of the collision response in place of the grid's
+ *              central differences (12 trilinear lookups); the network is
+ *              trained on the distance texture (sph_scene_grid_samples).  The distance itself, which
  *              switches the wall term (distWall < r0, cu:422) and sets a
  *              penalty of stiffness rho/dt^2, and the wall density and
  *              viscosity (which depend on each wall's ghost lattice, not on
@@ -624,7 +629,14 @@ const char* sph_mlp_last_error_message(const sph_mlp* m);
 typedef struct sph_neural_config {
     float tolerance;   /* validation bar, metres (e.g. 0.05 h) */
     float band;        /* |sdf| below which the bar applies (e.g. 2 h) */
-    float active;      /* reserved (0) */
+    /* bar on the wall normal (0 = not checked): RMS over the validation
+     * samples of a brick in the wall-penalty zone (label D > -r0) of
+     * w |G_network - G_grid|, the two central differences at +-spacing in
+     * world units (the network's normal and the grid's that it replaces;
+     * an exact signed distance has |G| = 1), w = min((r0 + D)/r0, 1) the
+     * penetration that scales the penalty force (cu:422-445): the error of
+     * the penalty force relative to its value at a penetration of r0 */
+    float normal_tolerance;
     int32_t brick;     /* validation brick edge, cells */
 } sph_neural_config;
 typedef struct sph_neural_report {
@@ -632,8 +644,10 @@ typedef struct sph_neural_report {
     uint32_t bricks_checked, bricks_fallback;
     float max_error, mean_error;   /* over validation samples within band */
     /* a brick falls back when the RMS error of its samples within band
-     * exceeds tolerance */
+     * exceeds tolerance, or the RMS normal error exceeds normal_tolerance */
     uint64_t samples_in_band;
+    float mean_normal_error;       /* mean w |G_network - G_grid| */
+    uint32_t bricks_fallback_normal;   /* of bricks_fallback: normal only */
 } sph_neural_report;
 /* val_xyzd: n_val validation samples (x, y, z, sdf), e.g. from
  * sph_scene_sdf_samples with another seed than the training set. */

@@ -129,6 +129,10 @@ def test_batch_scenes_with_learned_boundary_500_steps_against_oracle(oracle_traj
     for i, sc in enumerate(scenes):
         net = N.train_for_scene(sc.desc, steps=2000, n_samples=1 << 20)
         rep = b.attach_neural(i, net, val=N.validation_samples(sc.desc, 1 << 18))
+        print(f"scene {PICK[i]}: cells network {rep['cells_neural']}, grid {rep['cells_grid']}; bricks "
+              f"{rep['bricks_checked']} checked, {rep['bricks_fallback']} on the grid "
+              f"({rep['bricks_fallback_normal']} for the normal); mean error {rep['mean_error'] / sc.desc.h:.3f} h, "
+              f"normal {rep['mean_normal_error']:.3f}", flush=True)
         assert rep["cells_neural"] > 0
         nets.append(net)
     b.set_tuning(hit_masks=1, hit_list=40)

@@ -372,6 +372,10 @@ TUNINGS = [
     dict(hit_masks=1, hit_list=40),
     dict(hit_masks=1, hit_list=48),
     dict(hit_masks=1, hit_list=64),
+    # 128-thread workgroups (one-lane and hit-mask variants)
+    dict(density_lanes=1, force_lanes=1, block=128),
+    dict(hit_masks=1, hit_list=40, block=128),
+    dict(hit_masks=1, hit_list=64, block=128),
 ]
 
 
@@ -398,7 +402,8 @@ def test_kernel_variants_one_step_parity(dam16k, tuning):
     assert np.max(np.abs(sim.positions() - orc.read(Field.POSITION))) <= 1e-6
 
 
-@pytest.mark.parametrize("tuning", [TUNINGS[1], TUNINGS[2], TUNINGS[4], TUNINGS[6], TUNINGS[7], TUNINGS[9]], ids=_tid)
+@pytest.mark.parametrize("tuning", [TUNINGS[1], TUNINGS[2], TUNINGS[4], TUNINGS[6], TUNINGS[7], TUNINGS[9],
+                                    TUNINGS[10], TUNINGS[12]], ids=_tid)
 def test_kernel_variants_dense_cluster(dam16k, tuning):
     """Hundreds of neighbours per particle: the cooperative force kernel's
     per-particle hit list overflows (those particles take the direct loop),
