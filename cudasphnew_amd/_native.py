@@ -110,6 +110,15 @@ class SphBoundary(C.Structure):
     ]
 
 
+class SphDeviceBoundary(C.Structure):
+    """sph_device_boundary: the boundary grids in device memory (grid =
+    float4 per node, distance = float per node); pointers as integers."""
+    _fields_ = [
+        ("origin", f3), ("spacing", C.c_float), ("dims", i3),
+        ("rest_distance", C.c_float), ("grid", C.c_void_p), ("distance", C.c_void_p),
+    ]
+
+
 class SphScene(C.Structure):
     _fields_ = [
         ("num_particles", C.c_uint32), ("positions", fptr), ("velocities", fptr),
@@ -308,6 +317,8 @@ SIGNATURES = [
                                     C.POINTER(C.c_uint8)]),
     ("sph_abi_version", C.c_int32, []),
     ("sph_device_count", C.c_int32, []),
+    ("sph_step_noalloc", _st, [C.POINTER(SphParams), C.POINTER(SphDeviceBoundary), C.c_uint32, C.c_void_p,
+                               C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
 ]
 
 for _name, _res, _args in SIGNATURES:

@@ -52,6 +52,7 @@ SOURCES = [
     ("sim.hip", "device"),
     ("neural.hip", "device"),
     ("mesh.hip", "device"),
+    ("noalloc.hip", "device"),
 ]
 
 

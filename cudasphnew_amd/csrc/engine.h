@@ -138,6 +138,9 @@ struct Engine {
 };
 
 DevScene make_dev_scene(const sph_params& P, const sph_boundary& B);
+// The message sph_last_error_message(NULL) returns on this thread (errors of
+// calls without a handle: sph_create, sph_step_noalloc).
+void set_global_error(const std::string& msg);
 
 // Two-scale pass (twoscale.hip).
 sph_status twoscale_create(Engine& e);

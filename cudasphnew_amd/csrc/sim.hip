@@ -774,6 +774,10 @@ struct sph_batch {
 
 static thread_local std::string g_create_error;
 
+namespace sph {
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace sph
+
 extern "C" {
 
 int32_t sph_abi_version(void) { return SPH_ABI_VERSION; }

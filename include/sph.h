@@ -298,7 +298,7 @@ size_t sph_num_cells(const sph_handle* h);
  * reference): device bytes owned by the handle. */
 size_t sph_device_bytes(const sph_handle* h);
 /* Message of the last error on this handle ("" if none); NULL handle gives
- * the last error of sph_create. */
+ * the last error of sph_create or sph_step_noalloc on this thread. */
 const char* sph_last_error_message(const sph_handle* h);
 
 /* ---- timing ------------------------------------------------------------
@@ -726,6 +726,52 @@ sph_status sph_boundary_map_load(const char* path, sph_boundary_map_config* cfg,
                                  int32_t dims[3], float lo[3], float hi[3],
                                  float* distance, float* normal,
                                  uint8_t* present);
+
+/* ---- allocation-free step ---------------------------------------------------
+ * No reference counterpart.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): n_steps of sph_step's plain step (hash, counting sort by cell,
+ * reorder, density/pressure, acceleration + integration [+ collision]) with
+ * sph_step's default kernel variants, on a particle set held in caller
+ * device memory.  The call allocates nothing, copies nothing between host
+ * and device and does not synchronise: kernels only, on `stream` (a
+ * hipStream_t; NULL = the null stream).  Every other buffer (keys, orders,
+ * sorted copies, cell table, sort counts: SPH_NOALLOC_SCRATCH_BYTES) lives in
+ * one static device array of the library, one per GPU, held until the
+ * process ends.  All calls on one GPU share it: each call clears what must
+ * start cleared, but calls that overlap on one GPU (two streams, two host
+ * threads) race and give undefined results.  The first call on a device
+ * looks the array's address up (not a stream operation): make it outside
+ * graph capture.
+ *   positions   device float4 [n] (x, y, z, w), 16-byte aligned; in/out
+ *   velocities  device float4 [n] (vx, vy, vz, w); in/out
+ *               (particle i stays at index i; after the call w of both is
+ *               the density of the last step)
+ *   boundary    the boundary grids of sph_boundary, in device memory:
+ *               grid = float4 per node (distance, wall density, wall
+ *               viscosity, unused), distance = float per node, x fastest
+ *   status_flags  optional device uint32: set to 0, then OR-ed with 1
+ *               (particle outside the cell grid) and 2 (non-finite state)
+ * Limits: n <= SPH_NOALLOC_MAX_PARTICLES, cells <= SPH_NOALLOC_MAX_CELLS,
+ * else SPH_INVALID_ARG (message via sph_last_error_message(NULL)).  The
+ * result equals sph_create + sph_step(n_steps) + reading the fields by
+ * original id, bit for bit. */
+#define SPH_NOALLOC_MAX_PARTICLES 262144u
+#define SPH_NOALLOC_MAX_CELLS 1048576u
+#define SPH_NOALLOC_SCRATCH_BYTES 39848960u
+typedef struct sph_device_boundary {
+    float origin[3];
+    float spacing;
+    int32_t dims[3];
+    float rest_distance;
+    const float* grid;              /* float4 per node */
+    const float* distance;          /* float per node */
+} sph_device_boundary;
+sph_status sph_step_noalloc(const sph_params* params,
+                            const sph_device_boundary* boundary,
+                            uint32_t num_particles, float* positions,
+                            float* velocities, int32_t n_steps,
+                            uint32_t* status_flags, void* stream);
 
 /* ---- library info ----------------------------------------------------- */
 int32_t sph_abi_version(void);
