@@ -24,6 +24,69 @@ constexpr int kBlock = 256;
 
 inline unsigned blocks_for(uint32_t n, int block) { return (n + block - 1) / block; }
 
+// Scene record of a slot.  ONE (a single scene, every handle's case): index
+// 0, a uniform address, so the record's fields are scalar loads; else the
+// binary search over the scene begins, per lane (vector loads).
+template <bool ONE>
+__device__ __forceinline__ const DevScene& scene_for(const DevScene* __restrict__ scenes,
+                                                     const uint32_t* __restrict__ begins, int nscenes, uint32_t s) {
+    return ONE ? scenes[0] : scenes[scene_of(begins, nscenes, s)];
+}
+
+// Calls run(S) with the scene record of the calling lane's slot s.  A
+// wavefront's slots are consecutive, so when its first and last slot lie in
+// one scene (all but the few wavefronts that straddle a scene boundary) that
+// scene is found with two searches on uniform slots and passed at a uniform
+// address (scalar loads of its fields); otherwise per lane.  The branch is
+// uniform over the wavefront.
+template <bool UNIFORM>
+struct SceneAt {
+    // the uniform case reads the record at a readfirstlane index: scalar
+    // loads, and two instantiations the compiler cannot merge back into one
+    // body with a divergent address
+    __device__ __forceinline__ static const DevScene& get(const DevScene* __restrict__ scenes, int idx) {
+        return UNIFORM ? scenes[__builtin_amdgcn_readfirstlane(idx)] : scenes[idx];
+    }
+};
+
+// The same for kernels without cross-lane operations: one call site inside
+// a loop, uniform over the wavefront, over the scenes its slots lie in (one
+// iteration but for wavefronts that straddle a scene boundary), each pass
+// with the lanes of that scene; the scene index is the uniform loop counter.
+template <bool ONE, class Run>
+__device__ __forceinline__ void each_scene(const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins,
+                                           int nscenes, uint32_t s, uint32_t n, Run&& run) {
+    if (ONE) {
+        run(scenes[0]);
+        return;
+    }
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(s - (threadIdx.x & 63u));
+    const uint32_t w1 = min(w0 + 63u, n - 1u);
+    const int a = scene_of(begins, nscenes, w0), b = scene_of(begins, nscenes, w1);
+    const int mine = a == b ? a : scene_of(begins, nscenes, s);
+    for (int u = a; u <= b; ++u) {
+        // An SGPR copy the compiler cannot equate with the lane's own scene
+        // index, so the record is read with scalar loads.
+        int su = u;
+        asm volatile("" : "+s"(su));
+        if (mine == u) run(scenes[su]);
+    }
+}
+
+template <bool ONE, class Run>
+__device__ __forceinline__ void with_scene(const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins,
+                                           int nscenes, uint32_t s, uint32_t n, Run&& run) {
+    if (ONE) {
+        run(SceneAt<true>::get(scenes, 0));
+        return;
+    }
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(s - (threadIdx.x & 63u));
+    const uint32_t w1 = min(w0 + 63u, n - 1u);
+    const int a = scene_of(begins, nscenes, w0), b = scene_of(begins, nscenes, w1);
+    if (a == b) run(SceneAt<true>::get(scenes, a));
+    else run(SceneAt<false>::get(scenes, scene_of(begins, nscenes, s)));
+}
+
 // compute_particle_hash (particle_simulation.cu:89-119) for the storage
 // order.  Used before the first step; later keys come from the force kernel.
 __global__ __launch_bounds__(kBlock) void k_hash(const float4* __restrict__ pos, uint32_t* __restrict__ key,
@@ -303,7 +366,7 @@ __device__ __forceinline__ float wall_density(const DevScene& S, const float4* _
 // compute_particle_density_pressure (cu:196-256, 1385-1419), one thread per
 // particle.  Density goes to vel.w; pressure k(rho - rho0) is recomputed
 // where needed.
-template <int B>
+template <int B, bool ONE>
 __global__ __launch_bounds__(B) void k_density(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                     const int2* __restrict__ range, uint32_t n,
                                                     const DevScene* __restrict__ scenes,
@@ -311,7 +374,7 @@ __global__ __launch_bounds__(B) void k_density(const float4* __restrict__ pos, f
                                                     const float4* __restrict__ grid4, NeuralRefs nr) {
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const DevScene& S = scene_for<ONE>(scenes, begins, nscenes, s);
     float4 xi = pos[s];
     const float h2 = S.h2;
     Rows R = neighbour_rows(S, xi);
@@ -390,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_density_coop(const float4* __restric
 // test records the hits in a 64-bit mask, then the pair terms run for the
 // set bits only: roughly one candidate in six is within h, so a fused loop
 // would run the pair body for almost every candidate of a wavefront.
-template <int B>
+template <int B, bool ONE>
 __global__ __launch_bounds__(B) void k_force(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
@@ -400,7 +463,7 @@ __global__ __launch_bounds__(B) void k_force(
     uint32_t* __restrict__ key_out, uint32_t* __restrict__ status, NeuralRefs nr) {
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
+    const DevScene& S = scene_for<ONE>(scenes, begins, nscenes, s);
     const float4 xi = pos[s];
     const float4 vi = vel[s];
     const float rhoi = vi.w;
@@ -626,7 +689,7 @@ __device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi
 // wavefronts per SIMD) 131 us, 640 124 us, 512 121 us, 384 120 us, 256 135 us.
 constexpr int kWin = 384;
 
-template <int B>
+template <int B, bool ONE>
 __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                      const int2* __restrict__ range, uint32_t n,
                                                      const DevScene* __restrict__ scenes,
@@ -642,58 +705,59 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
     const uint32_t s = live ? s0 : n - 1;
     const int lane = threadIdx.x & 63;
     float* win = s_win[threadIdx.x >> 6];
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
-    const float4 xi = pos[s];
-    const float h2 = S.h2;
-    Rows R = neighbour_rows(S, xi);
-    int rb[9], re[9], wofs[9];
-    bool inlds[9];
-    int fill = 0;
-#pragma unroll
-    for (int row = 0; row < 9; ++row) {
-        row_slots(range, S, R, row / 3, row % 3, xi, rb[row], re[row]);
-        const bool any = re[row] > rb[row];
-        const int lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(any ? rb[row] : 0x7fffffff));
-        const int hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(any ? re[row] : -1));
-        inlds[row] = hi > lo && hi - lo <= kWin - fill;
-        wofs[row] = fill - lo;   // LDS index of slot q: q + wofs
-        if (inlds[row]) {
-            for (int i = lane; i < hi - lo; i += 64) {
-                const float4 p = pos[lo + i];
-                win[fill + i] = p.x;
-                win[kWin + fill + i] = p.y;
-                win[2 * kWin + fill + i] = p.z;
+    with_scene<ONE>(scenes, begins, nscenes, s, n, [&](const DevScene& S) {
+        const float4 xi = pos[s];
+        const float h2 = S.h2;
+        Rows R = neighbour_rows(S, xi);
+        int rb[9], re[9], wofs[9];
+        bool inlds[9];
+        int fill = 0;
+    #pragma unroll
+        for (int row = 0; row < 9; ++row) {
+            row_slots(range, S, R, row / 3, row % 3, xi, rb[row], re[row]);
+            const bool any = re[row] > rb[row];
+            const int lo = __builtin_amdgcn_readfirstlane(__ockl_wfred_min_i32(any ? rb[row] : 0x7fffffff));
+            const int hi = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(any ? re[row] : -1));
+            inlds[row] = hi > lo && hi - lo <= kWin - fill;
+            wofs[row] = fill - lo;   // LDS index of slot q: q + wofs
+            if (inlds[row]) {
+                for (int i = lane; i < hi - lo; i += 64) {
+                    const float4 p = pos[lo + i];
+                    win[fill + i] = p.x;
+                    win[kWin + fill + i] = p.y;
+                    win[2 * kWin + fill + i] = p.z;
+                }
+                fill += hi - lo;
             }
-            fill += hi - lo;
         }
-    }
-    // the window is private to the wavefront: its own LDS writes, in
-    // program order, are visible to its later reads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    float sum = 0.0f;
-#pragma unroll
-    for (int row = 0; row < 9; ++row) {
-        const int b = rb[row], cnt = re[row] - rb[row];
-        uint32_t m;
-        if (inlds[row]) {
-            const float* w = win + (b + wofs[row]);
-            m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
-                            sum);
-        } else {
-            m = density_row([&](int t) { const float4 p = pos[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
-                            xi, h2, sum);
+        // the window is private to the wavefront: its own LDS writes, in
+        // program order, are visible to its later reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float sum = 0.0f;
+    #pragma unroll
+        for (int row = 0; row < 9; ++row) {
+            const int b = rb[row], cnt = re[row] - rb[row];
+            uint32_t m;
+            if (inlds[row]) {
+                const float* w = win + (b + wofs[row]);
+                m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
+                                sum);
+            } else {
+                m = density_row([&](int t) { const float4 p = pos[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
+                                xi, h2, sum);
+            }
+            // the particle's own slot is not a force neighbour
+            const uint32_t self = s - (uint32_t)b;
+            if (self < (uint32_t)min(cnt, 32)) m &= ~(1u << self);
+            if (live) masks[(size_t)row * n + s] = cnt <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kLongRow;
         }
-        // the particle's own slot is not a force neighbour
-        const uint32_t self = s - (uint32_t)b;
-        if (self < (uint32_t)min(cnt, 32)) m &= ~(1u << self);
-        if (live) masks[(size_t)row * n + s] = cnt <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kLongRow;
-    }
-    if (live) {
-        float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
-        reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
-    }
+        if (live) {
+            float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
+            reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
+        }
+    });
 }
 
 // The hit list holds 16-bit entries (row << 6 | position in the row's run)
@@ -704,11 +768,13 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
 // row and are evaluated afterwards from the masks, so that the expansion
 // loop carries no pair arithmetic; the pair loop keeps the gathers of the
 // next three neighbours in flight while it evaluates the current one.
-// Measured at 1M particles: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4)
-// 132 us, 32 at 6 wavefronts spills and takes 139 us (156 us with two
-// gathers in flight instead of three).
-template <int CAP, int B>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
+// Measured at 1M particles before the packed row words and the scalar scene
+// loads: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4) 132 us, 32 at 6
+// wavefronts spills and takes 139 us (156 us with two gathers in flight
+// instead of three).  The batch form keeps its scene loop's state in a few
+// more registers and asks for four wavefronts at CAP 40 (five spill).
+template <int CAP, int B, bool ONE>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? (ONE ? 5 : 4) : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
     const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
@@ -722,91 +788,92 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5
     __shared__ int s_base[9][B];
     uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const DevScene& S = scenes[scene_of(begins, nscenes, s)];
-    uint64_t mk[9];
-#pragma unroll
-    for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
-    const float4 xi = pos[s];
-    const float4 vi = vel[s];
-    const float rhoi = vi.w;
-    const float h = S.h, h2 = S.h2;
-    const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
-    const float kr0 = S.stiff * S.rho0;
-    PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int cnt = 0;
-    uint32_t long_rows = 0;
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int row = 0; row < 9; ++row) {
-        if (mk[row] == kLongRow) {
-            long_rows |= 1u << row;
-            continue;
-        }
-        s_base[row][tid] = (int)(mk[row] >> 32);
-        uint32_t lo = (uint32_t)mk[row];
-        while (lo) {
-            const int t = __builtin_ctz(lo);
-            lo &= lo - 1;
-            s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
-            ++cnt;
-        }
-    }
-    const int m = min(cnt, CAP);
-    if (m > 0) {
-        // the gathers of entries k+1..k+3 are issued before the pair terms
-        // of entry k (short lists repeat their last entry, which is loaded
-        // but not evaluated)
-        auto slot = [&](int k) {
-            const int ent = s_list[min(k, m - 1)][tid];
-            return s_base[ent >> 6][tid] + (ent & 63);
-        };
-        int q = slot(0);
-        float4 x0 = pos[q], v0 = vel[q];
-        q = slot(1);
-        float4 x1 = pos[q], v1 = vel[q];
-        q = slot(2);
-        float4 x2 = pos[q], v2 = vel[q];
-        for (int k = 3; k < m; ++k) {
-            q = slot(k);
-            const float4 xn = pos[q], vn = vel[q];
-            pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
-            x0 = x1; v0 = v1;
-            x1 = x2; v1 = v2;
-            x2 = xn; v2 = vn;
-        }
-        pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
-        if (m > 1) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
-        if (m > 2) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
-    }
-    if (cnt > CAP || long_rows) {
-        // rare: hit lists longer than the capacity, rows longer than 32
-        const Rows R = neighbour_rows(S, xi);
-        int k = 0;
-#pragma unroll 1
+    each_scene<ONE>(scenes, begins, nscenes, s, n, [&](const DevScene& S) {
+        uint64_t mk[9];
+    #pragma unroll
+        for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
+        const float4 xi = pos[s];
+        const float4 vi = vel[s];
+        const float rhoi = vi.w;
+        const float h = S.h, h2 = S.h2;
+        const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
+        const float kr0 = S.stiff * S.rho0;
+        PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        int cnt = 0;
+        uint32_t long_rows = 0;
+        const int tid = threadIdx.x;
+    #pragma unroll
         for (int row = 0; row < 9; ++row) {
-            int b, e;
-            if (long_rows >> row & 1u) {
-                row_slots(range, S, R, row / 3, row % 3, xi, b, e);
-                for (int q = b; q < e; ++q) {
-                    float4 xj = pos[q];
-                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    float r2 = dx * dx + dy * dy + dz * dz;
-                    if (r2 <= h2 && r2 > 0.0f) pair_terms<false>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
-                }
+            if (mk[row] == kLongRow) {
+                long_rows |= 1u << row;
                 continue;
             }
-            b = s_base[row][tid];
-            uint32_t mm = (uint32_t)masks[(size_t)row * n + s];   // reloaded: keeps mk out of the pair loop
-            while (mm) {
-                const int t = __builtin_ctz(mm);
-                mm &= mm - 1;
-                if (k >= CAP) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
-                ++k;
+            s_base[row][tid] = (int)(mk[row] >> 32);
+            uint32_t lo = (uint32_t)mk[row];
+            while (lo) {
+                const int t = __builtin_ctz(lo);
+                lo &= lo - 1;
+                s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
+                ++cnt;
             }
         }
-    }
-    force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
-                 status, nr);
+        const int m = min(cnt, CAP);
+        if (m > 0) {
+            // the gathers of entries k+1..k+3 are issued before the pair terms
+            // of entry k (short lists repeat their last entry, which is loaded
+            // but not evaluated)
+            auto slot = [&](int k) {
+                const int ent = s_list[min(k, m - 1)][tid];
+                return s_base[ent >> 6][tid] + (ent & 63);
+            };
+            int q = slot(0);
+            float4 x0 = pos[q], v0 = vel[q];
+            q = slot(1);
+            float4 x1 = pos[q], v1 = vel[q];
+            q = slot(2);
+            float4 x2 = pos[q], v2 = vel[q];
+            for (int k = 3; k < m; ++k) {
+                q = slot(k);
+                const float4 xn = pos[q], vn = vel[q];
+                pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+                x0 = x1; v0 = v1;
+                x1 = x2; v1 = v2;
+                x2 = xn; v2 = vn;
+            }
+            pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+            if (m > 1) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
+            if (m > 2) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
+        }
+        if (cnt > CAP || long_rows) {
+            // rare: hit lists longer than the capacity, rows longer than 32
+            const Rows R = neighbour_rows(S, xi);
+            int k = 0;
+    #pragma unroll 1
+            for (int row = 0; row < 9; ++row) {
+                int b, e;
+                if (long_rows >> row & 1u) {
+                    row_slots(range, S, R, row / 3, row % 3, xi, b, e);
+                    for (int q = b; q < e; ++q) {
+                        float4 xj = pos[q];
+                        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                        float r2 = dx * dx + dy * dy + dz * dz;
+                        if (r2 <= h2 && r2 > 0.0f) pair_terms<false>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
+                    }
+                    continue;
+                }
+                b = s_base[row][tid];
+                uint32_t mm = (uint32_t)masks[(size_t)row * n + s];   // reloaded: keeps mk out of the pair loop
+                while (mm) {
+                    const int t = __builtin_ctz(mm);
+                    mm &= mm - 1;
+                    if (k >= CAP) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
+                    ++k;
+                }
+            }
+        }
+        force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
+                     status, nr);
+    });
 }
 
 // Surface particles (the reference's declared extractSurfaceParticles, on
@@ -909,16 +976,23 @@ void launch_reorder(hipStream_t st, const StepBuffers& b) {
                        b.slot_of_oid);
 }
 
+template <int B, bool ONE>
+static void launch_density_masks(hipStream_t st, const StepBuffers& b) {
+    hipLaunchKernelGGL((k_density_masks<B, ONE>), dim3(blocks_for(b.n, B)), dim3(B), 0, st, b.pos_b, b.vel_b,
+                       b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.masks, b.nr);
+}
+
+template <int B, bool ONE>
+static void launch_density_one_lane(hipStream_t st, const StepBuffers& b) {
+    hipLaunchKernelGGL((k_density<B, ONE>), dim3(blocks_for(b.n, B)), dim3(B), 0, st, b.pos_b, b.vel_b, b.range,
+                       b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr);
+}
+
 void launch_density(hipStream_t st, const StepBuffers& b, int lanes, int block) {
-    const bool small = block == 128;
-    dim3 g(blocks_for(b.n, small ? 128 : kBlock)), t(small ? 128 : kBlock);
+    const bool small = block == 128, one = b.nscenes == 1;
     if (b.masks) {
-        if (small)
-            hipLaunchKernelGGL(k_density_masks<128>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
-                               b.nscenes, b.grid4, b.masks, b.nr);
-        else
-            hipLaunchKernelGGL(k_density_masks<kBlock>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes,
-                               b.begins, b.nscenes, b.grid4, b.masks, b.nr);
+        if (small) one ? launch_density_masks<128, true>(st, b) : launch_density_masks<128, false>(st, b);
+        else one ? launch_density_masks<kBlock, true>(st, b) : launch_density_masks<kBlock, false>(st, b);
         return;
     }
     // the cooperative variants keep 256-thread blocks
@@ -929,12 +1003,8 @@ void launch_density(hipStream_t st, const StepBuffers& b, int lanes, int block) 
     case 8: hipLaunchKernelGGL(k_density_coop<8>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
     case 16: hipLaunchKernelGGL(k_density_coop<16>, gc, tc, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.nr); break;
     default:
-        if (small)
-            hipLaunchKernelGGL(k_density<128>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
-                               b.nscenes, b.grid4, b.nr);
-        else
-            hipLaunchKernelGGL(k_density<kBlock>, g, t, 0, st, b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins,
-                               b.nscenes, b.grid4, b.nr);
+        if (small) one ? launch_density_one_lane<128, true>(st, b) : launch_density_one_lane<128, false>(st, b);
+        else one ? launch_density_one_lane<kBlock, true>(st, b) : launch_density_one_lane<kBlock, false>(st, b);
         break;
     }
 }
@@ -959,23 +1029,23 @@ void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, cons
     hipLaunchKernelGGL(k_export_positions, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, st, n, pos, vel, oid, out);
 }
 
-template <int B>
+template <int B, bool ONE>
 static void launch_force_masks(hipStream_t st, const StepBuffers& b, int cap) {
     dim3 g(blocks_for(b.n, B)), t(B);
     switch (cap) {
-    case 32: hipLaunchKernelGGL((k_force_masks<32, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
-    case 64: hipLaunchKernelGGL((k_force_masks<64, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
-    case 40: hipLaunchKernelGGL((k_force_masks<40, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
-    default: hipLaunchKernelGGL((k_force_masks<48, B>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    case 32: hipLaunchKernelGGL((k_force_masks<32, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    case 64: hipLaunchKernelGGL((k_force_masks<64, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    case 40: hipLaunchKernelGGL((k_force_masks<40, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
+    default: hipLaunchKernelGGL((k_force_masks<48, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
     }
 }
 
 void launch_force(hipStream_t st, const StepBuffers& b, int lanes, int block) {
-    const bool small = block == 128;
+    const bool small = block == 128, one = b.nscenes == 1;
     if (b.masks) {
         // lanes selects the LDS hit-list capacity of the hit-mask variant
-        if (small) launch_force_masks<128>(st, b, lanes);
-        else launch_force_masks<kBlock>(st, b, lanes);
+        if (small) one ? launch_force_masks<128, true>(st, b, lanes) : launch_force_masks<128, false>(st, b, lanes);
+        else one ? launch_force_masks<kBlock, true>(st, b, lanes) : launch_force_masks<kBlock, false>(st, b, lanes);
         return;
     }
     dim3 g(blocks_for(b.n, kBlock)), t(kBlock);
@@ -985,10 +1055,13 @@ void launch_force(hipStream_t st, const StepBuffers& b, int lanes, int block) {
     case 8: hipLaunchKernelGGL(k_force_coop<8>, g, t, 0, st, SPH_FORCE_ARGS); break;
     case 16: hipLaunchKernelGGL(k_force_coop<16>, g, t, 0, st, SPH_FORCE_ARGS); break;
     default:
-        if (small)
-            hipLaunchKernelGGL(k_force<128>, dim3(blocks_for(b.n, 128)), dim3(128), 0, st, SPH_FORCE_ARGS);
-        else
-            hipLaunchKernelGGL(k_force<kBlock>, g, t, 0, st, SPH_FORCE_ARGS);
+        if (small) {
+            if (one) hipLaunchKernelGGL((k_force<128, true>), dim3(blocks_for(b.n, 128)), dim3(128), 0, st, SPH_FORCE_ARGS);
+            else hipLaunchKernelGGL((k_force<128, false>), dim3(blocks_for(b.n, 128)), dim3(128), 0, st, SPH_FORCE_ARGS);
+        } else {
+            if (one) hipLaunchKernelGGL((k_force<kBlock, true>), g, t, 0, st, SPH_FORCE_ARGS);
+            else hipLaunchKernelGGL((k_force<kBlock, false>), g, t, 0, st, SPH_FORCE_ARGS);
+        }
         break;
     }
 }

@@ -194,7 +194,10 @@ def test_learned_boundary_around_a_mesh_obstacle():
     print(f"grid: closest {grid[0]:.4f}, touching {grid[1]}; learned: closest {learned[0]:.4f}, "
           f"touching {learned[1]}; report {rep}; sources {src}")
     assert rep["mean_error"] < 0.05 * plain.h
-    assert rep["bricks_fallback"] <= 0.05 * rep["bricks_checked"]
+    # distance and penetration-weighted normal bars: a mesh's facets leave
+    # more bricks on the grid than an analytic scene's
+    assert rep["bricks_fallback"] <= 0.15 * rep["bricks_checked"]
+    assert rep["mean_normal_error"] < 0.05
     assert src[1] > 0.2 * scene.n          # the network is the wall source of most wall particles
     assert learned[1] > 50 and learned[0] > RADIUS
     # the learned normals and distances differ from the grid's inside the
