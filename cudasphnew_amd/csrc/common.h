@@ -37,7 +37,8 @@ struct DevScene {
     float ghalf_x, ghalf_y, ghalf_z;     // (dims - 1) / 2
     float bspacing, r0;
     int bnx, bny, bnz;
-    uint32_t pad0;
+    uint32_t dist_shift;                  // distance of node i at gridd[i << dist_shift]: 0 for the
+                                          // float copy, 2 when gridd is the float4 grid (channel x)
     uint64_t grid_offset;                 // first node in the grid pool
     float restitution;
     int collision;
@@ -170,11 +171,14 @@ __device__ __forceinline__ float lerp1(float a, float b, float t) { return a + t
 template <class T>
 __device__ __forceinline__ T lerp3(const T* __restrict__ g, const DevScene& S, const TexCoord& c);
 
+// The distance grid: its own float copy, or channel x of the float4 grid
+// (S.dist_shift = 2) once a learned boundary has replaced the copy.
 template <>
 __device__ __forceinline__ float lerp3<float>(const float* __restrict__ g, const DevScene& S,
                                               const TexCoord& c) {
-    size_t sx = 1, sy = (size_t)S.bnx, sz = (size_t)S.bnx * S.bny;
-    const float* b = g + S.grid_offset;
+    const uint32_t sh = S.dist_shift;
+    size_t sx = (size_t)1 << sh, sy = (size_t)S.bnx << sh, sz = ((size_t)S.bnx * S.bny) << sh;
+    const float* b = g + (S.grid_offset << sh);
     float v000 = b[c.x0 * sx + c.y0 * sy + c.z0 * sz], v100 = b[c.x1 * sx + c.y0 * sy + c.z0 * sz];
     float v010 = b[c.x0 * sx + c.y1 * sy + c.z0 * sz], v110 = b[c.x1 * sx + c.y1 * sy + c.z0 * sz];
     float v001 = b[c.x0 * sx + c.y0 * sy + c.z1 * sz], v101 = b[c.x1 * sx + c.y0 * sy + c.z1 * sz];

@@ -61,7 +61,8 @@ struct Engine {
     DevScene* d_scenes = nullptr;
     uint32_t* d_begins = nullptr;
     float4* grid4 = nullptr;
-    float* gridd = nullptr;
+    float* gridd = nullptr;                // float distance copy, or grid4 itself when gridd_alias
+    bool gridd_alias = false;
     size_t grid_nodes = 0;
     float4 *pos_a = nullptr, *vel_a = nullptr, *pos_b = nullptr, *vel_b = nullptr, *acc = nullptr;
     uint32_t *oid_a = nullptr, *oid_b = nullptr, *key = nullptr, *skey = nullptr, *perm = nullptr;
@@ -119,6 +120,9 @@ struct Engine {
 
     ~Engine();
     void drop_graph();
+    sph_status upload_scenes();
+    sph_status alias_distance();   // release the float distance copy (every scene has a network)
+    sph_status own_distance();     // allocate and fill it again from grid4
     sph_status set_tuning(const sph_tuning& t);
     sph_status autotune(int reps, sph_tuning* out);
     sph_status init(int S, const sph_params* P, const sph_scene* scenes);
@@ -162,7 +166,9 @@ sph_status twoscale_export(Engine& e, float4* out, sph_render_view* view);
 // Learned boundary (neural.hip).
 sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, size_t nval,
                          const sph_neural_config& cfg, sph_neural_report* rep);
-void neural_destroy(Engine& e);
+// Frees the learned boundary; with restore, gives the engine its float
+// distance copy back (it was released while every scene had a network).
+sph_status neural_destroy(Engine& e, bool restore = true);
 void neural_enqueue(Engine& e);
 size_t neural_bytes(const Engine& e);
 const float4* neural_records(const Engine& e);   // per sorted slot, or null

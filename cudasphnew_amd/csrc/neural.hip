@@ -911,9 +911,9 @@ struct NeuralBoundary {
     size_t bytes = 0;
 };
 
-void neural_destroy(Engine& e) {
+sph_status neural_destroy(Engine& e, bool restore) {
     NeuralBoundary* nb = e.nb;
-    if (!nb) return;
+    if (!nb) return SPH_OK;
     (void)hipStreamSynchronize(e.stream);
     void* ptrs[] = {nb->rec, nb->need, nb->list, nb->cnt, nb->d_enabled, nb->nets, nb->tiles};
     for (void* p : ptrs)
@@ -924,6 +924,7 @@ void neural_destroy(Engine& e) {
     (void)hipStreamSynchronize(e.stream);
     delete nb;
     e.nb = nullptr;
+    return restore ? e.own_distance() : SPH_OK;
 }
 
 size_t neural_bytes(const Engine& e) { return e.nb ? e.nb->bytes : 0; }
@@ -981,7 +982,7 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
                   hipMalloc(&nb->nets, sizeof(nn::NetView) * e.nscenes) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
-            neural_destroy(e);
+            (void)neural_destroy(e);
             return e.fail(SPH_OUT_OF_MEMORY, "learned boundary buffers");
         }
         nb->bytes = (sizeof(float4) + 1 + sizeof(uint32_t)) * (size_t)e.N;
@@ -1074,6 +1075,10 @@ sph_status neural_attach(Engine& e, int scene, sph_mlp* m, const float* val, siz
         return e.fail(SPH_DEVICE_ERROR, "network flag upload");
     st = rebuild_tiles(e);
     if (st != SPH_OK) return st;
+    // every scene's normals come from a network: the float distance copy goes
+    bool all = true;
+    for (int s = 0; s < e.nscenes; ++s) all = all && nb->enabled[s];
+    if (all && (st = e.alias_distance()) != SPH_OK) return st;
     e.drop_graph();
     return hipStreamSynchronize(e.stream) == hipSuccess ? SPH_OK : e.fail(SPH_DEVICE_ERROR, "attach sync");
 }

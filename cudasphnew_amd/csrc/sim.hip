@@ -106,10 +106,10 @@ static bool boundary_valid(const sph_boundary& B, std::string& why) {
 
 Engine::~Engine() {
     if (ts) twoscale_destroy(*this);
-    neural_destroy(*this);
+    neural_destroy(*this, false);
     drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-    void* ptrs[] = {d_scenes, d_begins, grid4, gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
+    void* ptrs[] = {d_scenes, d_begins, grid4, gridd_alias ? nullptr : gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
                     perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode,
                     render_pos, render_sub, surface};
     for (void* p : ptrs)
@@ -465,6 +465,7 @@ sph_status Engine::upload_params(int s, const sph_params& p) {
     n.grid_offset = d.grid_offset;
     n.p_begin = d.p_begin;
     n.p_end = d.p_end;
+    n.dist_shift = d.dist_shift;
     hscenes[s] = n;
     params[s] = p;
     params[s].num_particles = d.p_end - d.p_begin;
@@ -473,6 +474,64 @@ sph_status Engine::upload_params(int s, const sph_params& p) {
     if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "parameter upload sync");
     drop_graph();
     return SPH_OK;
+}
+
+namespace {
+
+__global__ __launch_bounds__(256) void k_distance_copy(const float4* __restrict__ grid4, uint64_t nodes,
+                                                       float* __restrict__ gridd) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nodes; i += (uint64_t)gridDim.x * blockDim.x)
+        gridd[i] = grid4[i].x;
+}
+
+}  // namespace
+
+sph_status Engine::upload_scenes() {
+    if (hipMemcpyAsync(d_scenes, hscenes.data(), nscenes * sizeof(DevScene), hipMemcpyHostToDevice, stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+        return fail(SPH_DEVICE_ERROR, "scene upload failed");
+    return SPH_OK;
+}
+
+// The float copy of the distance grid serves the central-difference normals
+// and the collision distance (4-byte texels, a quarter of the float4 grid's
+// lookup bytes).  Once every scene's normals come from a network, only the
+// cells that fall back to the grid read it, and they read channel x of the
+// float4 grid instead: the same values, so results do not change.
+sph_status Engine::alias_distance() {
+    if (gridd_alias) return SPH_OK;
+    if (hipStreamSynchronize(stream) != hipSuccess) return fail(SPH_DEVICE_ERROR, "sync before freeing the distance copy");
+    (void)hipFree(gridd);
+    bytes -= sizeof(float) * (grid_nodes ? grid_nodes : 1);
+    gridd = reinterpret_cast<float*>(grid4);
+    gridd_alias = true;
+    for (DevScene& d : hscenes) d.dist_shift = 2;
+    drop_graph();
+    return upload_scenes();
+}
+
+sph_status Engine::own_distance() {
+    if (!gridd_alias) return SPH_OK;
+    float* p = nullptr;
+    const size_t nb = sizeof(float) * (grid_nodes ? grid_nodes : 1);
+    if (hipMalloc(&p, nb) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SPH_OUT_OF_MEMORY, "distance copy");
+    }
+    const uint64_t blocks = std::min<uint64_t>((grid_nodes + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_distance_copy, dim3((unsigned)std::max<uint64_t>(blocks, 1)), dim3(256), 0, stream, grid4,
+                       (uint64_t)grid_nodes, p);
+    if (hipGetLastError() != hipSuccess) {
+        (void)hipFree(p);
+        return fail(SPH_DEVICE_ERROR, "distance copy kernel");
+    }
+    bytes += nb;
+    gridd = p;
+    gridd_alias = false;
+    for (DevScene& d : hscenes) d.dist_shift = 0;
+    drop_graph();
+    return upload_scenes();
 }
 
 void Engine::drop_graph() {
@@ -1026,9 +1085,9 @@ sph_status sph_attach_neural_boundary(sph_handle* h, sph_mlp* net, const float* 
 
 sph_status sph_detach_neural_boundary(sph_handle* h) {
     if (!h) return SPH_INVALID_ARG;
-    sph::neural_destroy(h->e);
+    sph_status st = sph::neural_destroy(h->e);
     h->e.drop_graph();
-    return SPH_OK;
+    return st;
 }
 
 sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t) {

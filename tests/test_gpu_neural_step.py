@@ -37,7 +37,40 @@ def test_attach_report(dam16k_net):
     assert rep["cells_neural"] > 0 and rep["cells_none"] > 0
     assert rep["samples_in_band"] > 10000
     assert rep["mean_error"] < 0.05 * sc.desc.h
-    assert rep["bricks_fallback"] <= 0.05 * rep["bricks_checked"]
+    # bricks fall back where the distance (0.05 h) or the penetration-weighted
+    # normal (0.05) misses its bar; the network serves most wall cells
+    assert rep["bricks_fallback"] <= 0.15 * rep["bricks_checked"]
+    assert rep["cells_neural"] > 2 * rep["cells_grid"]
+    assert rep["mean_normal_error"] < 0.05
+
+
+def test_attach_releases_the_distance_copy(dam16k_net):
+    """With a network on every scene the float copy of the distance grid is
+    freed (4 bytes per node; the cells that fall back to the grid read the
+    float4 grid's distance channel); detaching allocates it again from that
+    channel, and the plain step is unchanged."""
+    sc, net = dam16k_net
+    nodes = int(np.prod(sc.boundary_dims))
+    n = sc.params.num_particles
+    a = S.ParticleSimulation(sc).Init()
+    b = S.ParticleSimulation(sc).Init()
+    b0 = a.GetSizeMemoryGPU()
+    a.attach_neural(net)
+    b1 = a.GetSizeMemoryGPU()
+    print(f"device bytes: grid form {b0}, learned boundary {b1}; {nodes} nodes, {n} particles")
+    assert b0 - b1 == 4 * nodes - 21 * n   # distance copy out, per-particle records (16 + 1 + 4 B) in
+    a.Advance(5)
+    a.detach_neural()
+    assert a.GetSizeMemoryGPU() == b0
+    c = S.ParticleSimulation(sc).Init()
+    c.attach_neural(net)
+    c.detach_neural()
+    c.Advance(20)
+    b.Advance(20)
+    c.sync()
+    b.sync()
+    for f in (Field.POSITION, Field.VELOCITY, Field.DENSITY):
+        assert np.array_equal(c.read(f), b.read(f))
 
 
 def test_far_and_grid_cells_follow_the_grid_form(dam16k_net):
@@ -110,9 +143,10 @@ def test_detach_restores_grid_form(dam16k_net):
 def test_network_is_a_compact_stand_in_for_the_1m_distance_grid():
     """What the network gains over the grid: at the 1M-particle scene it
     holds the distance texture in 16,641 parameters (66.6 KB in FP32) in
-    place of 256x128x256 nodes (33.5 MB), with its validation RMS error
-    within 0.05 h in all but a few bricks, and the step that uses its
-    normals keeps mean density and kinetic energy with the grid form."""
+    place of 256x128x256 nodes (33.5 MB), with its validation RMS distance
+    error within 0.05 h and its normal within the penetration-weighted bar
+    in most bricks, and the step that uses its normals keeps mean density and
+    kinetic energy with the grid form."""
     sc = S.make_scene("dam1m")
     h = float(sc.desc.h)
     net = N.train_for_scene(sc.desc)
@@ -123,7 +157,9 @@ def test_network_is_a_compact_stand_in_for_the_1m_distance_grid():
     print(f"grid {grid_bytes / 1e6:.1f} MB vs network {net_bytes / 1e3:.1f} KB ({grid_bytes / net_bytes:.0f}x); {rep}")
     assert grid_bytes > 400 * net_bytes
     assert rep["mean_error"] < 0.05 * h
-    assert rep["bricks_fallback"] <= 0.05 * rep["bricks_checked"]
+    assert rep["bricks_fallback"] <= 0.15 * rep["bricks_checked"]
+    assert rep["cells_neural"] > 2 * rep["cells_grid"]
+    assert rep["mean_normal_error"] < 0.05
     ref = S.ParticleSimulation(sc).Init()
     sim.Advance(200)
     ref.Advance(200)
