@@ -8,7 +8,10 @@
 //            (cu:730-740, 932-1017); the boundary bit is pushed from the
 //            split particles as in the reference, but into a separate final
 //            state array: every writer of a byte stores the same value
-//            (state | 2) instead of the reference's racy byte OR.
+//            (state | 2) instead of the reference's racy byte OR.  With the
+//            hit-mask pair the push follows the coarse density pass's row
+//            words (the split particle's neighbours within h) instead of
+//            searching the cell table.
 //   collect  check_split_boundary_default + 3 exclusive scans + collect_ids
 //            (cu:1092-1162, 2509-2567) as one 3-way compaction over
 //            original ids; counts stay on the device (the reference copies
@@ -22,7 +25,9 @@
 //            the stable radix sort for sub grids too large for it.
 //   physics  sub density (split subs), sub acceleration (split subs), sub
 //            integration + collision (all subs) (cu:259-313, 477-568,
-//            667-696, 836-930).
+//            667-696, 836-930); the density pass records each split sub's
+//            neighbours as packed row words and the acceleration pass runs
+//            its pair terms over them.
 // Every kernel reads the live counts from device memory; grids are sized
 // for the sub-particle capacity.
 #include <algorithm>
@@ -35,7 +40,8 @@ namespace sph {
 
 struct TSCounts {
     uint32_t nsplit, nboundary, ndefault, nsub, nparents, nsub_prev;
-    uint32_t nmark;   // entries of the split list of this step (k_ts_split); zeroed by k_ts_force
+    uint32_t nmark;   // entries of the split list of this step (k_ts_split or the hit-mask
+                      // density pass); zeroed by k_ts_force
     uint32_t pad1;
 };
 
@@ -63,6 +69,7 @@ struct TwoScale {
     CellSort csort;
     bool use_cell_sort = false;
     uint32_t* sperm_raw = nullptr;
+    uint64_t* smasks = nullptr;   // [9][cap] packed row words of the split subs (sub density -> sub force)
     size_t bytes = 0;
     bool sub_cells_valid = false;
 };
@@ -168,6 +175,65 @@ __global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, 
                 const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
                 if (dx * dx + dy * dy + dz * dz <= h2) state_fin[q] = (uint8_t)(sq | 2);
             }
+        }
+    }
+}
+
+// The same push driven by the coarse density pass's packed row words (first
+// slot << 32 | mask of the row's candidates within h, own slot excluded;
+// ~0 for a row of more than 32 candidates), when the step runs the hit-mask
+// pair: one lane per split particle and row, no cell-table reads or distance
+// tests for all but long rows.
+constexpr uint64_t kCoarseLongRow = ~0ull;
+
+__global__ __launch_bounds__(kB) void k_ts_mark_rows(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
+                                                     const int2* __restrict__ range, const DevScene* __restrict__ scenes,
+                                                     const TSCounts* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ mark_list,
+                                                     const uint64_t* __restrict__ masks, uint32_t n,
+                                                     uint8_t* __restrict__ state_fin) {
+    const uint64_t total = 9ull * cnt->nmark;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += stride) {
+        const uint32_t s = mark_list[g / 9];
+        const int row = (int)(g % 9);
+        if (row == 0) state_fin[s] = (uint8_t)(state[s] | 2);   // within h of itself
+        const uint64_t w = masks[(size_t)row * n + s];
+        if (w != kCoarseLongRow) {
+            const uint32_t b = (uint32_t)(w >> 32);
+            uint32_t m = (uint32_t)w;
+            while (m) {
+                const uint32_t q = b + (uint32_t)(__ffs(m) - 1);
+                m &= m - 1;
+                const uint8_t sq = state[q];
+                if (!(sq & 1)) state_fin[q] = (uint8_t)(sq | 2);
+            }
+            continue;
+        }
+        // long row: its candidates tested here
+        const DevScene& S = scenes[0];
+        const float h = S.h, h2 = S.h2;
+        const float4 xi = pos[s];
+        const int lx = range_coord(xi.x, -h, S.ox, S.cell, S.nx), hx = range_coord(xi.x, h, S.ox, S.cell, S.nx);
+        const int ly = range_coord(xi.y, -h, S.oy, S.cell, S.ny), hy = range_coord(xi.y, h, S.oy, S.cell, S.ny);
+        const int lz = range_coord(xi.z, -h, S.oz, S.cell, S.nz), hz = range_coord(xi.z, h, S.oz, S.cell, S.nz);
+        const int k = lz + row / 3, j = ly + row % 3;
+        if (k > hz || j > hy) continue;
+        const uint32_t rowbase = S.cell_offset + (uint32_t)(S.nx * (S.ny * k + j));
+        int b = 0x7fffffff, e = -1;
+        for (int i = lx; i <= hx; ++i) {
+            const int2 r = range[rowbase + (uint32_t)i];
+            if (r.x >= 0) {
+                b = min(b, r.x);
+                e = max(e, r.y);
+            }
+        }
+        for (int q = b; q < e; ++q) {
+            const uint8_t sq = state[q];
+            if (sq & 1) continue;
+            const float4 xj = pos[q];
+            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            if (dx * dx + dy * dy + dz * dz <= h2) state_fin[q] = (uint8_t)(sq | 2);
         }
     }
 }
@@ -489,12 +555,56 @@ __device__ __forceinline__ float group_sum(float v) {
     return v;
 }
 
+// The nine x-row runs of a sub-particle's neighbourhood on the sub grid: the
+// sub grid's cells (size cell_sub), support h_sub, the same 3 x 3 row set
+// and y-z cull as the coarse passes.
+struct SubRows {
+    int lx, hx, ly, hy, lz, hz;
+};
+
+__device__ __forceinline__ SubRows sub_rows(const DevScene& S, float4 x) {
+    SubRows r;
+    sub_range(S, x, S.h_sub, r.lx, r.hx, r.ly, r.hy, r.lz, r.hz);
+    return r;
+}
+
+__device__ __forceinline__ void sub_row_slots(const int2* __restrict__ srange, const DevScene& S, const SubRows& R,
+                                              int dk, int dj, float4 xi, int& b, int& e) {
+    const int k = R.lz + dk, j = R.ly + dj;
+    b = 0;
+    e = 0;
+    if (k > R.hz || j > R.hy) return;
+    const float y0 = S.oy + (float)j * S.cell_sub, z0 = S.oz + (float)k * S.cell_sub;
+    const float dy = fmaxf(fmaxf(y0 - xi.y, xi.y - (y0 + S.cell_sub)), 0.0f);
+    const float dz = fmaxf(fmaxf(z0 - xi.z, xi.z - (z0 + S.cell_sub)), 0.0f);
+    if ((dy * dy + dz * dz) * 0.99999f > S.h2_sub) return;
+    const uint32_t rowbase = (uint32_t)(S.snx * (S.sny * k + j));
+    b = 0x7fffffff;
+    e = -1;
+    for (int i = R.lx; i <= R.hx; ++i) {
+        const int2 r = srange[rowbase + (uint32_t)i];
+        if (r.x >= 0) {
+            b = min(b, r.x);
+            e = max(e, r.y);
+        }
+    }
+    if (e < b) { b = 0; e = 0; }
+}
+
+// Row word of the sub passes: first slot << 32 | force-neighbour mask of the
+// row's first 32 candidates, or kSubLongRow for a row of more candidates.
+constexpr uint64_t kSubLongRow = ~0ull;
+
 // compute_sub_particle_density_pressure (cu:259-313): split subs only, no
-// wall term.
+// wall term.  Besides the density, each split sub's group records per row the
+// candidates within h_sub (r^2 <= h_sub^2, its own slot excluded) as a
+// packed row word, so that the force pass runs its pair terms over those
+// bits without distance tests or cell-table reads.
 __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                    const float4* __restrict__ spos_s, float4* __restrict__ svel_s,
                                                    const int2* __restrict__ srange,
-                                                   const DevScene* __restrict__ scenes) {
+                                                   const DevScene* __restrict__ scenes, uint32_t cap,
+                                                   uint64_t* __restrict__ smasks) {
     const uint64_t total = (uint64_t)cnt->nsub * kTsLanes;
     const uint32_t nsplit8 = 8u * cnt->nsplit;
     const DevScene& S = scenes[0];
@@ -505,28 +615,42 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
         const int lane = (int)(g % kTsLanes);
         const bool split = sperm[t] < nsplit8;   // boundary subs: projected density
         float sum = 0.0f;
-        if (split) {
-            const float4 xi = spos_s[t];
-            int lx, hx, ly, hy, lz, hz;
-            sub_range(S, xi, S.h_sub, lx, hx, ly, hy, lz, hz);
-            for (int k = lz; k <= hz; ++k)
-                for (int j = ly; j <= hy; ++j) {
-                    uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
-                    int b = 0x7fffffff, e = -1;
-                    for (int i = lx; i <= hx; ++i) {
-                        int2 r = srange[row + (uint32_t)i];
-                        if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
-                    }
-                    // (an empty row leaves b = INT_MAX: b + lane must not be formed)
-                    if (b >= e) continue;
+        const float4 xi = spos_s[t];
+        const SubRows R = sub_rows(S, xi);
+#pragma unroll 1
+        for (int row = 0; row < 9; ++row) {
+            int b = 0, e = 0;
+            if (split) sub_row_slots(srange, S, R, row / 3, row % 3, xi, b, e);
+            // the sign bits of h_sub^2 - r^2 of this lane's candidates
+            // b + lane + 4k, newest in bit 0 (as in the coarse density pass)
+            uint32_t far = 0;
 #pragma unroll 4
-                    for (int q = b + lane; q < e; q += kTsLanes) {
-                        float4 xj = spos_s[q];
-                        float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-                        float d = fmaxf(h2 - (dx * dx + dy * dy + dz * dz), 0.0f);
-                        sum += d * d * d;
-                    }
-                }
+            for (int q = b + lane; q < e; q += kTsLanes) {
+                const float4 xj = spos_s[q];
+                const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+                float d = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2)));
+                far = __builtin_amdgcn_alignbit(far, __float_as_uint(d), 31);
+                d = fmaxf(d, 0.0f);
+                sum += d * d * d;
+            }
+            // a row of at most 32 candidates gives each lane at most 8: bit k
+            // of the lane's near bits goes to candidate position 4k + lane
+            uint32_t m = 0;
+            if (e - b <= 32) {
+                const int c = e - b > lane ? (e - b - lane + kTsLanes - 1) / kTsLanes : 0;
+                uint32_t x = c > 0 ? __builtin_bitreverse32(~far) >> (32 - c) : 0u;
+                x = (x | (x << 12)) & 0x000F000Fu;
+                x = (x | (x << 6)) & 0x03030303u;
+                x = (x | (x << 3)) & 0x11111111u;
+                m = x << lane;
+            }
+            // every lane of the group takes part in the shuffles
+            m |= __shfl_xor(m, 1, 64);
+            m |= __shfl_xor(m, 2, 64);
+            const uint32_t self = t - (uint32_t)b;   // the sub's own slot is no neighbour
+            if (self < 32u) m &= ~(1u << self);
+            if (split && lane == 0)
+                smasks[(size_t)row * cap + t] = e - b <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kSubLongRow;
         }
         sum = group_sum(sum);   // every lane of the group takes part
         if (split && lane == 0) reinterpret_cast<float*>(svel_s)[4 * (size_t)t + 3] = sum * (S.poly6_sub * S.mass_sub);
@@ -536,17 +660,14 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
 // compute_sub_particle_acceleration (cu:477-568) for split subs (boundary
 // subs: zero acceleration), then integrate_sub_particles_euler (cu:667-696)
 // and the collision response for every sub-particle.  One thread per
-// sub-particle.  Each row run is first reduced to a 64-bit mask of the
-// candidates within h (cheap: the distance test only), and the pair terms
-// then run over the set bits: a wavefront executes the pair terms as often
-// as its lane with the most neighbours in the row, not once per candidate
-// that any lane hits.  Rows over 64 candidates take the direct loop for the
-// rest.  The terms are added in the same candidate order as a direct loop.
-// (Four lanes per sub-particle measured slower: 65 us without masks, 58 us
-// with a mask per lane, against 42 us.)
+// sub-particle.  The pair terms run over the set bits of the density pass's
+// row words, in candidate order (rows in z-major, y-minor order, slots
+// ascending), so a wavefront executes them as often as its lane with the most
+// neighbours in the row; a row of more than 32 candidates is tested here.
 __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                  const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
                                                  const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
+                                                 const uint64_t* __restrict__ smasks, uint32_t cap,
                                                  const float* __restrict__ gridd, float4* __restrict__ spos_out,
                                                  float4* __restrict__ svel_out, float4* __restrict__ sacc,
                                                  uint32_t* __restrict__ status) {
@@ -569,17 +690,18 @@ __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, con
         const float rhoi = vi.w;
         const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;
         const float kr0 = S.stiff * S.rho0;
-        int lx, hx, ly, hy, lz, hz;
-        sub_range(S, xi, h, lx, hx, ly, hy, lz, hz);
         float fpx = 0, fpy = 0, fpz = 0, fvx = 0, fvy = 0, fvz = 0, cgx = 0, cgy = 0, cgz = 0, cl = 0;
         auto pair = [&](int q) {
             float4 xj = spos_s[q];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             float r2 = dx * dx + dy * dy + dz * dz;
             float4 vj = svel_s[q];
-            float inv_r = __builtin_amdgcn_rsqf(r2);
+            // a distinct sub at distance 0 contributes nothing (every term
+            // carries 1/r or 1/rho_j, both taken as 0 there)
+            const bool nz = r2 > 0.0f;
+            float inv_r = nz ? __builtin_amdgcn_rsqf(r2) : 0.0f;
             float rn = r2 * inv_r;
-            float invj = __builtin_amdgcn_rcpf(vj.w);
+            float invj = nz ? __builtin_amdgcn_rcpf(vj.w) : 0.0f;
             float pj = S.stiff * vj.w - kr0;
             float hr = h - rn, qd = h2 - r2;
             float cp = (Ai + rhoi * pj * invj * invj) * hr * hr * inv_r;
@@ -596,28 +718,24 @@ __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, con
             const float r2 = dx * dx + dy * dy + dz * dz;
             return r2 <= h2 && r2 > 0.0f;
         };
-        for (int k = lz; k <= hz; ++k)
-            for (int j = ly; j <= hy; ++j) {
-                uint32_t row = (uint32_t)(S.snx * (S.sny * k + j));
-                int b = 0x7fffffff, e = -1;
-                for (int i = lx; i <= hx; ++i) {
-                    int2 r = srange[row + (uint32_t)i];
-                    if (r.x >= 0) { b = min(b, r.x); e = max(e, r.y); }
-                }
-                if (b >= e) continue;   // empty row
-                const int c64 = min(e - b, 64);
-                uint64_t m = 0;
-#pragma unroll 4
-                for (int u = 0; u < c64; ++u)
-                    if (near(b + u)) m |= 1ull << u;
+#pragma unroll 1
+        for (int row = 0; row < 9; ++row) {
+            const uint64_t w = smasks[(size_t)row * cap + t];
+            if (w != kSubLongRow) {
+                const int b = (int)(w >> 32);
+                uint32_t m = (uint32_t)w;
                 while (m) {
-                    const int u = __ffsll((unsigned long long)m) - 1;
+                    const int u = __ffs(m) - 1;
                     m &= m - 1;
                     pair(b + u);
                 }
-                for (int q = b + 64; q < e; ++q)
+            } else {
+                int b, e;
+                sub_row_slots(srange, S, sub_rows(S, xi), row / 3, row % 3, xi, b, e);
+                for (int q = b; q < e; ++q)
                     if (near(q)) pair(q);
             }
+        }
         const float cpf = -S.mass_sub * S.grad_spiky_sub, cvf = S.mu * S.mass_sub * S.lapl_visc_sub;
         float Fx = cvf * fvx + cpf * fpx, Fy = cvf * fvy + cpf * fpy, Fz = cvf * fvz + cpf * fpz;
         float cm = S.mass_sub * S.grad_poly6_sub;
@@ -726,7 +844,8 @@ sph_status twoscale_create(Engine& e) {
               ts_alloc(*t, &t->sskey, t->cap) == hipSuccess && ts_alloc(*t, &t->sperm, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->spos_s, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_s, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->spos_out, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_out, t->cap) == hipSuccess &&
-              ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess;
+              ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess &&
+              ts_alloc(*t, &t->smasks, 9 * (size_t)t->cap) == hipSuccess;
     t->use_cell_sort = (uint64_t)t->sub_cells <= (uint64_t)kCellSortRatio * t->cap;
     if (ok && t->use_cell_sort)
         ok = ts_alloc(*t, &t->sperm_raw, t->cap) == hipSuccess && t->csort.init(t->sub_cells, t->cap) == hipSuccess;
@@ -751,7 +870,7 @@ void twoscale_destroy(Engine& e) {
     void* ptrs[] = {t->state_fin, t->cls,   t->split_ids, t->boundary_ids, t->default_ids, t->blk,
                     t->counts,    t->spos,  t->svel,      t->sid,          t->skey,        t->sskey,
                     t->sperm,     t->spos_s, t->svel_s,   t->spos_out,     t->svel_out,    t->sacc,
-                    t->srange,    t->sperm_raw, t->mark_list};
+                    t->srange,    t->sperm_raw, t->mark_list, t->smasks};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     t->sorter.release();
@@ -780,20 +899,31 @@ static void sub_pass(Engine& e, TwoScale& t, bool keys_ready) {
     e.mark(kStageTwoScaleSubSort);
     const unsigned sub_grid = (unsigned)std::min<uint64_t>(kTsGrid, ((uint64_t)t.cap * kTsLanes + kB - 1) / kB);
     hipLaunchKernelGGL(k_ts_density, dim3(sub_grid), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
-                       t.srange, e.d_scenes);
+                       t.srange, e.d_scenes, t.cap, t.smasks);
     hipLaunchKernelGGL(k_ts_force, dim3(nblocks(t.cap)), dim3(kB), 0, st, t.counts, t.sperm, t.spos_s, t.svel_s,
-                       t.srange, e.d_scenes, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
+                       t.srange, e.d_scenes, t.smasks, t.cap, e.gridd, t.spos_out, t.svel_out, t.sacc, e.status);
     e.mark(kStageTwoScaleSubPhysics);
 }
 
-void twoscale_enqueue(Engine& e) {
+SplitRefs twoscale_split_refs(Engine& e) {
+    TwoScale& t = *e.ts;
+    return SplitRefs{e.state_b, t.state_fin, &t.counts->nmark, t.mark_list};
+}
+
+void twoscale_enqueue(Engine& e, const uint64_t* masks) {
     TwoScale& t = *e.ts;
     hipStream_t st = e.stream;
     const uint32_t n = e.N;
-    hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, t.state_fin, n,
-                       e.d_scenes, t.counts, t.mark_list);
-    hipLaunchKernelGGL(k_ts_mark, dim3(kMarkGrid), dim3(kB), 0, st, e.pos_b, e.state_b, e.range, e.d_scenes,
-                       t.counts, t.mark_list, t.state_fin);
+    // with the hit-mask pair the density pass has set the split flags
+    if (!masks)
+        hipLaunchKernelGGL(k_ts_split, dim3(nblocks(n)), dim3(kB), 0, st, e.pos_b, e.state_b, t.state_fin, n,
+                           e.d_scenes, t.counts, t.mark_list);
+    if (masks)
+        hipLaunchKernelGGL(k_ts_mark_rows, dim3(kMarkGrid), dim3(kB), 0, st, e.pos_b, e.state_b, e.range,
+                           e.d_scenes, t.counts, t.mark_list, masks, n, t.state_fin);
+    else
+        hipLaunchKernelGGL(k_ts_mark, dim3(kMarkGrid), dim3(kB), 0, st, e.pos_b, e.state_b, e.range, e.d_scenes,
+                           t.counts, t.mark_list, t.state_fin);
     hipLaunchKernelGGL(k_ts_classify, dim3(nblocks(n)), dim3(kB), 0, st, t.state_fin, e.oid_b, n, t.cls);
     e.mark(kStageTwoScaleState);
     hipLaunchKernelGGL(k_ts_count, dim3(t.nblk), dim3(kB), 0, st, t.cls, n, t.blk, t.nblk);
