@@ -219,16 +219,19 @@ def cpu_baseline(scene, seconds: float, threads: int):
     }
 
 
-def kernel_name(stage: str, tuning: dict) -> str:
-    """The kernel a stage launched under the given variants."""
+def kernel_name(stage: str, tuning: dict, one_scene: bool = True) -> str:
+    """The kernel (as rocprofv3 names it, template arguments included) that a
+    stage launched under the given variants."""
+    b, one = tuning["block"], "true" if one_scene else "false"
     if stage == "force":
         if tuning["hit_masks"]:
-            return f"k_force_masks<{tuning['hit_list']}>"
-        return "k_force" if tuning["force_lanes"] == 1 else f"k_force_coop<{tuning['force_lanes']}>"
+            return f"k_force_masks<{tuning['hit_list']},{b},{one}>"
+        return f"k_force<{b},{one}>" if tuning["force_lanes"] == 1 else f"k_force_coop<{tuning['force_lanes']}>"
     if stage == "density":
         if tuning["hit_masks"]:
-            return "k_density_masks"
-        return "k_density" if tuning["density_lanes"] == 1 else f"k_density_coop<{tuning['density_lanes']}>"
+            return f"k_density_masks<{b},{one}>"
+        return f"k_density<{b},{one}>" if tuning["density_lanes"] == 1 else \
+            f"k_density_coop<{tuning['density_lanes']}>"
     return f"k_{stage}"
 
 
@@ -461,7 +464,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-steps", type=int, default=50, help="timed steps of the batched forms")
     ap.add_argument("--tuning", default="auto",
                     help="density/force kernel variants: 'auto' (autotune after the warm-up) or "
-                         "'D,F,M,C' = density lanes, force lanes, hit masks (0/1), hit-list capacity")
+                         "'D,F,M,C[,B]' = density lanes, force lanes, hit masks (0/1), hit-list capacity, "
+                         "workgroup size (128 or 256; default 256)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -472,13 +476,19 @@ def main(argv=None) -> int:
 
     if a.train_profile:
         return train_profile(S, a)
+    if a.copy_probe:
+        copy_probe(256 << 20)
     if a.batch_profile:
         scenes = [S.make_scene("batch", index=i, seed=1) for i in range(64)]
         b = S.Batch(scenes)
         b.use_graph(True)
         b.step(3)
         b.sync()
-        tun = b.autotune(reps=2)
+        if a.tuning == "auto":
+            tun = b.autotune(reps=2)
+        else:
+            b.set_tuning(*(int(v) for v in a.tuning.split(",")))
+            tun = b.get_tuning()
         b.step(a.batch_profile)
         b.sync()
         log(f"batch profile: {a.batch_profile} steps, 64 scenes, {sum(x.n for x in scenes)} particles, {tun}")
@@ -494,8 +504,6 @@ def main(argv=None) -> int:
         log(f"two-scale profile: {a.two_scale_profile} steps, {sc2.n} coarse, {c2.num_split} split, "
             f"{c2.num_boundary} boundary, {c2.num_sub} sub-particles")
         return 0
-    if a.copy_probe:
-        copy_probe(256 << 20)
     t0 = time.time()
     sc = S.make_scene(a.config)
     log(f"scene {a.config}: {sc.n} particles, grid {tuple(sc.params.grid_dim)}, "

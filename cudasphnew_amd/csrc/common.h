@@ -52,6 +52,17 @@ struct DevScene {
 // from every surface), the network's per-particle record, or the grid.
 enum : uint8_t { kWallNone = 0, kWallNeural = 1, kWallGrid = 2 };
 
+// The two-scale state pass fused into the hit-mask density pass
+// (shift_state + find_split_particles, particle_simulation.cu:730-740,
+// 932-1001) on this step's sorted particles; state == null: no two-scale
+// step.
+struct SplitRefs {
+    uint8_t* state;        // per sorted slot: state << 2, split bit set here
+    uint8_t* state_fin;    // the final state's starting value (the boundary bit comes later)
+    uint32_t* nmark;       // length of the split list (zero before the pass)
+    uint32_t* mark_list;   // slots of the split particles, one atomic per wavefront
+};
+
 struct NeuralRefs {
     const uint8_t* mode;    // per global cell key; null: grid everywhere
     const uint32_t* skey;   // sorted keys

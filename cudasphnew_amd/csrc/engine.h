@@ -60,6 +60,8 @@ struct Engine {
     // device memory
     DevScene* d_scenes = nullptr;
     uint32_t* d_begins = nullptr;
+    uint2* d_tiles = nullptr;            // batch work list of the hit-mask kernels (nscenes > 1)
+    uint32_t ntiles = 0;
     float4* grid4 = nullptr;
     float* gridd = nullptr;                // float distance copy, or grid4 itself when gridd_alias
     bool gridd_alias = false;
@@ -149,7 +151,11 @@ void set_global_error(const std::string& msg);
 // Two-scale pass (twoscale.hip).
 sph_status twoscale_create(Engine& e);
 void twoscale_destroy(Engine& e);
-void twoscale_enqueue(Engine& e);
+// masks: the coarse density pass's row words when the step runs the hit-mask
+// pair (that pass then also set the split flags, from twoscale_split_refs),
+// or null
+void twoscale_enqueue(Engine& e, const uint64_t* masks);
+SplitRefs twoscale_split_refs(Engine& e);
 sph_status twoscale_read(Engine& e, sph_field f, void* dst, size_t bytes_out);
 sph_status twoscale_counts(Engine& e, sph_counts* out);
 size_t twoscale_bytes(const Engine& e);

@@ -14,6 +14,11 @@ struct StepBuffers {
     int nscenes = 0;
     const DevScene* scenes = nullptr;     // [nscenes]
     const uint32_t* begins = nullptr;     // [nscenes] first slot of each scene
+    // batch work list of the hit-mask kernels: per wavefront (first slot,
+    // scene << 7 | slot count), never across a scene boundary (nscenes > 1):
+    // the aligned 64-slot wavefronts, then the scenes' starts inside them
+    const uint2* tiles = nullptr;
+    uint32_t ntiles = 0;
     const float4* grid4 = nullptr;        // (distance, density, viscosity, 0) pool
     const float* gridd = nullptr;         // distance pool
     // storage order ("a") and cell-sorted order ("b")
@@ -35,6 +40,8 @@ struct StepBuffers {
     // per-row force-neighbour masks [9][n] written by the density pass and
     // read by the force pass; null = neighbour loops without masks
     uint64_t* masks = nullptr;
+    // two-scale split flags, set by the hit-mask density pass (state == null: none)
+    SplitRefs split = {nullptr, nullptr, nullptr, nullptr};
     // learned boundary records (mode == null: grid everywhere)
     NeuralRefs nr = {nullptr, nullptr, nullptr, nullptr, nullptr, 0};
 };

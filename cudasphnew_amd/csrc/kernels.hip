@@ -33,58 +33,44 @@ __device__ __forceinline__ const DevScene& scene_for(const DevScene* __restrict_
     return ONE ? scenes[0] : scenes[scene_of(begins, nscenes, s)];
 }
 
-// Calls run(S) with the scene record of the calling lane's slot s.  A
-// wavefront's slots are consecutive, so when its first and last slot lie in
-// one scene (all but the few wavefronts that straddle a scene boundary) that
-// scene is found with two searches on uniform slots and passed at a uniform
-// address (scalar loads of its fields); otherwise per lane.  The branch is
-// uniform over the wavefront.
-template <bool UNIFORM>
-struct SceneAt {
-    // the uniform case reads the record at a readfirstlane index: scalar
-    // loads, and two instantiations the compiler cannot merge back into one
-    // body with a divergent address
-    __device__ __forceinline__ static const DevScene& get(const DevScene* __restrict__ scenes, int idx) {
-        return UNIFORM ? scenes[__builtin_amdgcn_readfirstlane(idx)] : scenes[idx];
-    }
+// Slot of a lane in the hit-mask kernels.  ONE: consecutive slots, lanes
+// past the end repeat the last slot (not live).  Batch: the work list holds
+// one entry per wavefront (x = first slot, y = scene << 7 | slot count
+// 1..64), each inside one scene, so the wavefront's scene index is uniform
+// and its record is read with scalar loads.  The entries are built once per
+// batch from the scene ranges (the segmented sort keeps them fixed): first
+// the 64-slot-aligned wavefronts of the whole slot range, each running the
+// lanes of the scene its first slot belongs to, then, for the lanes of the
+// later scenes that start inside one of them, one entry each -- at most one
+// per scene, and every other wavefront's memory accesses stay aligned.
+// Returns false for a wavefront with no work.
+struct WaveSlot {
+    uint32_t s;
+    bool live;
+    int scene;
 };
 
-// The same for kernels without cross-lane operations: one call site inside
-// a loop, uniform over the wavefront, over the scenes its slots lie in (one
-// iteration but for wavefronts that straddle a scene boundary), each pass
-// with the lanes of that scene; the scene index is the uniform loop counter.
-template <bool ONE, class Run>
-__device__ __forceinline__ void each_scene(const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins,
-                                           int nscenes, uint32_t s, uint32_t n, Run&& run) {
+template <int B, bool ONE>
+__device__ __forceinline__ bool wave_slot(uint32_t n, const uint2* __restrict__ tiles, uint32_t ntiles,
+                                          WaveSlot& w) {
+    const uint32_t lane = threadIdx.x & 63u;
     if (ONE) {
-        run(scenes[0]);
-        return;
+        const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * B + threadIdx.x;
+        if ((s0 & ~63u) >= n) return false;
+        w.live = s0 < n;
+        w.s = w.live ? s0 : n - 1;
+        w.scene = 0;
+        return true;
     }
-    const uint32_t w0 = __builtin_amdgcn_readfirstlane(s - (threadIdx.x & 63u));
-    const uint32_t w1 = min(w0 + 63u, n - 1u);
-    const int a = scene_of(begins, nscenes, w0), b = scene_of(begins, nscenes, w1);
-    const int mine = a == b ? a : scene_of(begins, nscenes, s);
-    for (int u = a; u <= b; ++u) {
-        // An SGPR copy the compiler cannot equate with the lane's own scene
-        // index, so the record is read with scalar loads.
-        int su = u;
-        asm volatile("" : "+s"(su));
-        if (mine == u) run(scenes[su]);
-    }
-}
-
-template <bool ONE, class Run>
-__device__ __forceinline__ void with_scene(const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins,
-                                           int nscenes, uint32_t s, uint32_t n, Run&& run) {
-    if (ONE) {
-        run(SceneAt<true>::get(scenes, 0));
-        return;
-    }
-    const uint32_t w0 = __builtin_amdgcn_readfirstlane(s - (threadIdx.x & 63u));
-    const uint32_t w1 = min(w0 + 63u, n - 1u);
-    const int a = scene_of(begins, nscenes, w0), b = scene_of(begins, nscenes, w1);
-    if (a == b) run(SceneAt<true>::get(scenes, a));
-    else run(SceneAt<false>::get(scenes, scene_of(begins, nscenes, s)));
+    const uint32_t t =
+        __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (B / 64) + (threadIdx.x >> 6));
+    if (t >= ntiles) return false;
+    const uint2 e = tiles[t];
+    const uint32_t cnt = e.y & 127u;
+    w.scene = (int)(e.y >> 7);
+    w.live = lane < cnt;
+    w.s = e.x + (w.live ? lane : cnt - 1u);
+    return true;
 }
 
 // compute_particle_hash (particle_simulation.cu:89-119) for the storage
@@ -687,25 +673,31 @@ __device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi
 // kWin trades rows served from LDS against occupancy (the window is the
 // kernel's LDS): measured at 1M particles, 768 entries (all nine rows, four
 // wavefronts per SIMD) 131 us, 640 124 us, 512 121 us, 384 120 us, 256 135 us.
+// Three planes rather than float4 entries: one 12-byte read per candidate
+// without address arithmetic (float4 window) measured 114 us against 109 us
+// at 384 entries (117 us at 320, 122 us at 256) -- the larger window costs
+// more occupancy than the saved instructions gain.
 constexpr int kWin = 384;
 
 template <int B, bool ONE>
 __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ pos, float4* __restrict__ vel,
                                                      const int2* __restrict__ range, uint32_t n,
                                                      const DevScene* __restrict__ scenes,
-                                                     const uint32_t* __restrict__ begins, int nscenes,
+                                                     const uint2* __restrict__ tiles, uint32_t ntiles,
                                                      const float4* __restrict__ grid4,
-                                                     uint64_t* __restrict__ masks, NeuralRefs nr) {
+                                                     uint64_t* __restrict__ masks, NeuralRefs nr,
+                                                     SplitRefs split) {
     __shared__ float s_win[B / 64][3 * kWin];
-    const uint32_t s0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if ((s0 & ~63u) >= n) return;   // the whole wavefront lies past the end
+    WaveSlot ws;
+    if (!wave_slot<B, ONE>(n, tiles, ntiles, ws)) return;
     // every lane of a live wavefront takes part in the window copies; lanes
     // past the end repeat the last particle and store nothing
-    const bool live = s0 < n;
-    const uint32_t s = live ? s0 : n - 1;
+    const bool live = ws.live;
+    const uint32_t s = ws.s;
     const int lane = threadIdx.x & 63;
     float* win = s_win[threadIdx.x >> 6];
-    with_scene<ONE>(scenes, begins, nscenes, s, n, [&](const DevScene& S) {
+    {
+        const DevScene& S = scenes[ws.scene];
         const float4 xi = pos[s];
         const float h2 = S.h2;
         Rows R = neighbour_rows(S, xi);
@@ -757,7 +749,32 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
             float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
             reinterpret_cast<float*>(vel)[4 * (size_t)s + 3] = rho;
         }
-    });
+        if (split.state) {
+            // two-scale step: the particle's split flag (inside the split
+            // box) on top of its shifted state, and the split particles'
+            // slots appended to the split list (every lane takes part)
+            bool in = false;
+            if (live) {
+                uint8_t st = (uint8_t)(split.state[s] << 2);
+                in = xi.x >= S.split_lo[0] && xi.x <= S.split_hi[0] && xi.y >= S.split_lo[1] &&
+                     xi.y <= S.split_hi[1] && xi.z >= S.split_lo[2] && xi.z <= S.split_hi[2];
+                if (in) st |= 1;
+                split.state[s] = st;
+                split.state_fin[s] = st;
+            }
+            const uint64_t ins = __ballot(in);
+            if (ins) {
+                const int leader = __ffsll((unsigned long long)ins) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(split.nmark, (uint32_t)__popcll(ins));
+                base = __shfl(base, leader, 64);
+                if (in) {
+                    const uint64_t below = lane == 0 ? 0ull : (ins & (~0ull >> (64 - lane)));
+                    split.mark_list[base + (uint32_t)__popcll(below)] = s;
+                }
+            }
+        }
+    }
 }
 
 // The hit list holds 16-bit entries (row << 6 | position in the row's run)
@@ -771,12 +788,11 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
 // Measured at 1M particles before the packed row words and the scalar scene
 // loads: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4) 132 us, 32 at 6
 // wavefronts spills and takes 139 us (156 us with two gathers in flight
-// instead of three).  The batch form keeps its scene loop's state in a few
-// more registers and asks for four wavefronts at CAP 40 (five spill).
+// instead of three).
 template <int CAP, int B, bool ONE>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? (ONE ? 5 : 4) : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
-    const DevScene* __restrict__ scenes, const uint32_t* __restrict__ begins, int nscenes,
+    const DevScene* __restrict__ scenes, const uint2* __restrict__ tiles, uint32_t ntiles,
     const float4* __restrict__ grid4, const float* __restrict__ gridd, const uint32_t* __restrict__ oid_b,
     const uint8_t* __restrict__ state_b, float4* __restrict__ acc, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, uint32_t* __restrict__ oid_out, uint8_t* __restrict__ state_out,
@@ -786,9 +802,11 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? (
     // consecutive banks
     __shared__ uint16_t s_list[CAP + 1][B];
     __shared__ int s_base[9][B];
-    uint32_t s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    each_scene<ONE>(scenes, begins, nscenes, s, n, [&](const DevScene& S) {
+    WaveSlot ws;
+    if (!wave_slot<B, ONE>(n, tiles, ntiles, ws) || !ws.live) return;
+    const uint32_t s = ws.s;
+    {
+        const DevScene& S = scenes[ws.scene];
         uint64_t mk[9];
     #pragma unroll
         for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
@@ -873,7 +891,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? (
         }
         force_finish(S, s, xi, vi, a, grid4, gridd, oid_b, state_b, acc, pos_out, vel_out, oid_out, state_out, key_out,
                      status, nr);
-    });
+    }
 }
 
 // Surface particles (the reference's declared extractSurfaceParticles, on
@@ -976,10 +994,17 @@ void launch_reorder(hipStream_t st, const StepBuffers& b) {
                        b.slot_of_oid);
 }
 
+// Workgroups of the hit-mask kernels: one thread per slot, or (batch) one
+// wavefront per work-list entry.
+template <int B, bool ONE>
+static unsigned mask_blocks(const StepBuffers& b) {
+    return ONE ? blocks_for(b.n, B) : blocks_for(b.ntiles, B / 64);
+}
+
 template <int B, bool ONE>
 static void launch_density_masks(hipStream_t st, const StepBuffers& b) {
-    hipLaunchKernelGGL((k_density_masks<B, ONE>), dim3(blocks_for(b.n, B)), dim3(B), 0, st, b.pos_b, b.vel_b,
-                       b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.masks, b.nr);
+    hipLaunchKernelGGL((k_density_masks<B, ONE>), dim3(mask_blocks<B, ONE>(b)), dim3(B), 0, st, b.pos_b, b.vel_b,
+                       b.range, b.n, b.scenes, b.tiles, b.ntiles, b.grid4, b.masks, b.nr, b.split);
 }
 
 template <int B, bool ONE>
@@ -1014,7 +1039,7 @@ void launch_density(hipStream_t st, const StepBuffers& b, int lanes, int block) 
         b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.nr
 
 #define SPH_FORCE_ARGS_M                                                                                            \
-    b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.begins, b.nscenes, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,    \
+    b.pos_b, b.vel_b, b.range, b.n, b.scenes, b.tiles, b.ntiles, b.grid4, b.gridd, b.oid_b, b.state_b, b.acc,      \
         b.pos_a, b.vel_a, b.oid_a, b.state_a, b.key, b.status, b.masks, b.nr
 
 void launch_surface(hipStream_t st, const StepBuffers& b, float cm_thresh, float n_thresh, uint8_t* out) {
@@ -1031,7 +1056,7 @@ void launch_export_positions(hipStream_t st, uint32_t n, const float4* pos, cons
 
 template <int B, bool ONE>
 static void launch_force_masks(hipStream_t st, const StepBuffers& b, int cap) {
-    dim3 g(blocks_for(b.n, B)), t(B);
+    dim3 g(mask_blocks<B, ONE>(b)), t(B);
     switch (cap) {
     case 32: hipLaunchKernelGGL((k_force_masks<32, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;
     case 64: hipLaunchKernelGGL((k_force_masks<64, B, ONE>), g, t, 0, st, SPH_FORCE_ARGS_M); break;

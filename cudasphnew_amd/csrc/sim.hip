@@ -109,7 +109,7 @@ Engine::~Engine() {
     neural_destroy(*this, false);
     drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-    void* ptrs[] = {d_scenes, d_begins, grid4, gridd_alias ? nullptr : gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
+    void* ptrs[] = {d_scenes, d_begins, d_tiles, grid4, gridd_alias ? nullptr : gridd, pos_a, vel_a, pos_b, vel_b, acc, oid_a, oid_b, key, skey,
                     perm, slot_of_oid, state_a, state_b, range, status, n_dev, masks, wall_mode,
                     render_pos, render_sub, surface};
     for (void* p : ptrs)
@@ -152,6 +152,7 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
     }
     (void)hipGetDevice(&device);
     nscenes = S;
+    if (S > (1 << 24)) return fail(SPH_INVALID_ARG, "at most 2^24 scenes");
     params.assign(P, P + S);
     hscenes.resize(S);
     hbegins.assign(S + 1, 0);
@@ -200,6 +201,28 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
 #define SPH_TRY(x) do { if ((st = (x)) != SPH_OK) return st; } while (0)
     SPH_TRY(alloc(&d_scenes, S));
     SPH_TRY(alloc(&d_begins, S + 1));
+    // wavefront work list of the batch's hit-mask kernels, no entry across a
+    // scene boundary: the aligned 64-slot wavefronts (each for the scene of
+    // its first slot, up to that scene's end), then the slots of every scene
+    // that starts inside one of them, up to the next aligned slot
+    std::vector<uint2> htiles;
+    if (S > 1) {
+        const uint32_t n = (uint32_t)np;
+        int a = 0;
+        for (uint32_t f = 0; f < n; f += 64) {
+            while (hbegins[a + 1] <= f) ++a;   // the scene of slot f
+            const uint32_t cnt = std::min<uint32_t>(64u, hbegins[a + 1] - f);
+            htiles.push_back(make_uint2(f, (uint32_t)a << 7 | cnt));
+        }
+        for (int s = 0; s < S; ++s) {
+            const uint32_t b = hbegins[s], e = hbegins[s + 1];
+            if (b == e || b % 64 == 0) continue;
+            const uint32_t stop = std::min<uint32_t>(e, (b / 64 + 1) * 64);
+            htiles.push_back(make_uint2(b, (uint32_t)s << 7 | (stop - b)));
+        }
+        ntiles = (uint32_t)htiles.size();
+        SPH_TRY(alloc(&d_tiles, htiles.size()));
+    }
     SPH_TRY(alloc(&grid4, nodes));
     SPH_TRY(alloc(&wall_mode, total_cells));
     SPH_TRY(alloc(&gridd, nodes));
@@ -268,6 +291,7 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
         hipMemcpy(oid_a, ho.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_scenes, hscenes.data(), S * sizeof(DevScene), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_begins, hbegins.data(), (S + 1) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        (ntiles && hipMemcpy(d_tiles, htiles.data(), ntiles * sizeof(uint2), hipMemcpyHostToDevice) != hipSuccess) ||
         hipMemcpy(n_dev, &N, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
         return fail(SPH_DEVICE_ERROR, "state upload failed");
     // cell tables start empty (Init, cu:1944-1946); state bytes start at 0
@@ -292,6 +316,8 @@ StepBuffers Engine::buffers() const {
     b.nscenes = nscenes;
     b.scenes = d_scenes;
     b.begins = d_begins;
+    b.tiles = d_tiles;
+    b.ntiles = ntiles;
     b.grid4 = grid4;
     b.gridd = gridd;
     b.pos_a = pos_a; b.vel_a = vel_a; b.oid_a = oid_a; b.state_a = ts ? state_a : nullptr;
@@ -371,10 +397,12 @@ void Engine::enqueue_step(bool two_scale) {
     }
     launch_reorder(stream, b);
     mark(kStageReorder);
+    // a two-scale step's split flags come from the hit-mask density pass
+    if (two_scale && b.masks) b.split = twoscale_split_refs(*this);
     launch_density(stream, b, tuning.density_lanes, tuning.block);
     mark(kStageDensity);
     if (two_scale) {
-        twoscale_enqueue(*this);
+        twoscale_enqueue(*this, b.masks);
         b.state_b = twoscale_final_state(*this);
     }
     if (nb) {

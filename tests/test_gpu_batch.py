@@ -165,3 +165,29 @@ def test_full_batch_matches_handles_and_sorts():
         cs, ce = b.read(i, Field.CELL_START), b.read(i, Field.CELL_END)
         occ = cs >= 0
         assert np.array_equal(np.repeat(np.nonzero(occ)[0], (ce - cs)[occ]), sk)
+
+
+@pytest.mark.parametrize("batch_tuning", [dict(hit_list=40, block=128), dict(hit_list=48, block=256)],
+                         ids=lambda t: f"{t['hit_list']}-{t['block']}")
+def test_batch_hit_mask_work_list_matches_handles(batch_tuning):
+    """The batch's hit-mask pair runs one wavefront per work-list entry, none
+    across a scene boundary: the 64-slot-aligned wavefronts, then the lanes
+    of each scene that starts inside one.  Each particle's arithmetic is the
+    handle's, so every scene equals its own handle bit for bit over 3 steps
+    -- here the eight picked scenes plus two of fewer than 64 particles (a
+    scene inside one wavefront) placed between them."""
+    picked = _scenes(PICK)
+    tiny = [picked[0].with_positions(picked[0].positions[:30]), picked[3].with_positions(picked[3].positions[:7])]
+    scenes = picked[:2] + [tiny[0]] + picked[2:5] + [tiny[1]] + picked[5:]
+    b = S.Batch(scenes)
+    b.set_tuning(hit_masks=1, **batch_tuning)
+    b.step(3)
+    b.sync()
+    for i, sc in enumerate(scenes):
+        sim = S.ParticleSimulation(sc).Init()
+        sim.set_tuning(hit_masks=1, hit_list=40)
+        sim.Advance(3)
+        sim.sync()
+        for f in (Field.POSITION, Field.VELOCITY, Field.DENSITY, Field.ACCEL, Field.SORTED_IDS):
+            assert np.array_equal(b.read(i, f), sim.read(f)), (i, sc.n, f)
+        sim.close()
