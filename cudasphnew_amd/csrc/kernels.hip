@@ -596,6 +596,20 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
                  key_out, status, nr);
 }
 
+// Raw buffer view of a float4 array: a gather takes a 32-bit byte offset
+// (one shift, and one offset register shared by the position and velocity
+// gathers of a neighbour) instead of a 64-bit address per load.  Slots stay
+// below 2^28 (the engine's particle limit).  Measured at 1M particles: force
+// pass 95 → 91 us.
+struct F4Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ explicit F4Buf(const float4* p)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, -1, 0x00020000)) {}
+    __device__ __forceinline__ float4 operator[](uint32_t q) const {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(q * 16u), 0, 0));
+    }
+};
+
 // Hit-mask pipeline.  The density pass already evaluates the distance of
 // every candidate of the nine row runs; it records, per particle and row, the
 // row's first slot and which of its candidates are force neighbours, packed
@@ -737,7 +751,8 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
                 m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
                                 sum);
             } else {
-                m = density_row([&](int t) { const float4 p = pos[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
+                const F4Buf pb(pos);
+                m = density_row([&](int t) { const float4 p = pb[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
                                 xi, h2, sum);
             }
             // the particle's own slot is not a force neighbour
@@ -836,6 +851,7 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5
             }
         }
         const int m = min(cnt, CAP);
+        const F4Buf pb(pos), vb(vel);
         if (m > 0) {
             // the gathers of entries k+1..k+3 are issued before the pair terms
             // of entry k (short lists repeat their last entry, which is loaded
@@ -845,14 +861,14 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5
                 return s_base[ent >> 6][tid] + (ent & 63);
             };
             int q = slot(0);
-            float4 x0 = pos[q], v0 = vel[q];
+            float4 x0 = pb[q], v0 = vb[q];
             q = slot(1);
-            float4 x1 = pos[q], v1 = vel[q];
+            float4 x1 = pb[q], v1 = vb[q];
             q = slot(2);
-            float4 x2 = pos[q], v2 = vel[q];
+            float4 x2 = pb[q], v2 = vb[q];
             for (int k = 3; k < m; ++k) {
                 q = slot(k);
-                const float4 xn = pos[q], vn = vel[q];
+                const float4 xn = pb[q], vn = vb[q];
                 pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
                 x0 = x1; v0 = v1;
                 x1 = x2; v1 = v2;

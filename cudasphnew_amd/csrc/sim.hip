@@ -180,7 +180,8 @@ sph_status Engine::init(int S, const sph_params* P, const sph_scene* scenes) {
         hscenes[s].p_end = (uint32_t)np;
         nodes += (uint64_t)scenes[s].boundary.dims[0] * scenes[s].boundary.dims[1] * scenes[s].boundary.dims[2];
         if (ncells > 0x7fffffffull) return fail(SPH_INVALID_ARG, "total cells exceed 32-bit keys");
-        if (np > 0x7fffffffull) return fail(SPH_INVALID_ARG, "too many particles");
+        // the neighbour gathers address slots with 32-bit byte offsets
+        if (np >= (1ull << 28)) return fail(SPH_INVALID_ARG, "at most 268,435,455 particles per handle or batch");
     }
     hbegins[S] = (uint32_t)np;
     N = (uint32_t)np;
@@ -594,11 +595,12 @@ sph_status Engine::set_tuning(const sph_tuning& t_in) {
     return SPH_OK;
 }
 
-// Device time of one launch of the current density and force variants:
-// reps back-to-back launches of each between two events on the handle's
-// stream, on the last step's sorted state.  Both passes only read the sorted
-// buffers and rewrite what they wrote in that step with the same values, so
-// the state is unchanged.
+// Device time of one launch of the current density and force variants: reps
+// launches of each captured in a graph and replayed between two events on
+// the handle's stream (a graph runs its kernels back to back, without the
+// dispatch gap each eager launch adds), on the last step's sorted state.
+// Both passes only read the sorted buffers and rewrite what they wrote in
+// that step with the same values, so the state is unchanged.
 sph_status Engine::time_passes(int reps, float* density_ms, float* force_ms) {
     if (reps < 1 || !density_ms || !force_ms) return fail(SPH_INVALID_ARG, "reps >= 1 and two outputs required");
     if (!keys_valid) return fail(SPH_INVALID_ARG, "timing the passes needs the state of one step: advance once first");
@@ -615,16 +617,25 @@ sph_status Engine::time_passes(int reps, float* density_ms, float* force_ms) {
     }
     bool ok = true;
     auto time_it = [&](auto&& run) -> float {
-        run();   // warm-up
-        (void)hipEventRecord(e0, stream);
-        for (int r = 0; r < reps; ++r) run();
-        (void)hipEventRecord(e1, stream);
-        if (hipEventSynchronize(e1) != hipSuccess) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t gx = nullptr;
+        float ms = 0.0f;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             ok = false;
             return 0.0f;
         }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
+        for (int r = 0; r < reps; ++r) run();
+        if (hipStreamEndCapture(stream, &g) != hipSuccess || hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess ||
+            hipGraphLaunch(gx, stream) != hipSuccess ||   // warm-up
+            hipEventRecord(e0, stream) != hipSuccess || hipGraphLaunch(gx, stream) != hipSuccess ||
+            hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            (void)hipGetLastError();
+            ok = false;
+        } else {
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (gx) (void)hipGraphExecDestroy(gx);
+        if (g) (void)hipGraphDestroy(g);
         return ms / (float)reps;
     };
     *density_ms = time_it([&] { launch_density(stream, b, tuning.density_lanes, tuning.block); });
@@ -633,7 +644,7 @@ sph_status Engine::time_passes(int reps, float* density_ms, float* force_ms) {
     });
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!ok) return fail(SPH_DEVICE_ERROR, "timing the passes: kernel failed");
+    if (!ok) return fail(SPH_DEVICE_ERROR, "timing the passes: graph capture or replay failed");
     return check_launch("time passes");
 }
 

@@ -241,8 +241,9 @@ class ParticleSimulation:
         return float(ms.value)
 
     def time_neighbour_passes(self, reps: int = 20) -> tuple:
-        """(density ms, force ms) per launch: back-to-back launches of the
-        current variants on the last step's state (sph_time_neighbour_passes)."""
+        """(density ms, force ms) per launch: launches of the current variants
+        replayed from a graph on the last step's state
+        (sph_time_neighbour_passes)."""
         d, f = C.c_float(), C.c_float()
         check(lib.sph_time_neighbour_passes(self._need(), int(reps), C.byref(d), C.byref(f)), self._h)
         return float(d.value), float(f.value)

@@ -190,7 +190,9 @@ typedef struct sph_counts {
 /* ---- lifecycle ---------------------------------------------------------
  * sph_create replaces the factory + Init + Bind sequence
  * (particle_simulation.h:174, cu:1745, cu:1882, cu:2078).  Parameters are
- * validated; the handle owns copies of everything. */
+ * validated; the handle owns copies of everything.  At most 268,435,455
+ * particles per handle (or batch): the neighbour gathers address particles
+ * with 32-bit byte offsets. */
 sph_status sph_create(const sph_params* params, const sph_scene* scene,
                       sph_handle** out);
 /* ~ParticleSimulation (cu:1689-1735). */
@@ -324,11 +326,12 @@ sph_status sph_use_graph(sph_handle* h, int32_t enable);
 sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
                           float* ms);
 /* Device time of one launch of the current density and force variants:
- * reps back-to-back launches of each between two HIP events on the handle's
- * stream, on the state of the last step (needs one step first; the state is
- * left unchanged: both passes only read the sorted buffers and rewrite their
- * outputs with the same values).  No reference counterpart (measurement):
- * the kernel time without the launch gaps that per-stage events include. */
+ * reps launches of each captured in a hipGraph and replayed between two HIP
+ * events on the handle's stream, on the state of the last step (needs one
+ * step first; the state is left unchanged: both passes only read the sorted
+ * buffers and rewrite their outputs with the same values).  No reference
+ * counterpart (measurement): the kernel time without the dispatch gaps that
+ * eager launches and per-stage events include. */
 sph_status sph_time_neighbour_passes(sph_handle* h, int32_t reps,
                                      float* density_ms, float* force_ms);
 
