@@ -545,17 +545,6 @@ def main(argv=None) -> int:
     keys = sim.read(S.Field.SORTED_KEYS)
     tc = touched_cells(sc.params, keys)
     single = {k: v for k, v in stages.items() if k in STAGE_BYTES_PER_PARTICLE and v > 0}
-    # the neighbour passes' launch time without the dispatch gaps that an
-    # event around each eager stage includes: 20 launches of the chosen
-    # variants on the last step's state, replayed from a captured graph
-    # between HIP events on the handle's stream (the figure rocprofv3 reports
-    # as the kernel's duration)
-    d_ms, f_ms = sim.time_neighbour_passes(20)
-    log(f"neighbour passes, graph-replayed launches: density {d_ms * 1e3:.1f} us, force {f_ms * 1e3:.1f} us")
-    if "density" in single:
-        single["density"] = d_ms
-    if "force" in single:
-        single["force"] = f_ms
     dom = max(single, key=single.get)
     tex = touched_texels(sc, sim)
     algo = {k: STAGE_BYTES_PER_PARTICLE[k] * n for k in single}
@@ -581,8 +570,8 @@ def main(argv=None) -> int:
         "profiled_busy": profiled_busy(a.config, kname),
         "algorithmic_bytes_per_launch": int(algo_bytes),
         "avg_launch_ms": round(single[dom], 5),
-        "timing": "density/force: 20 launches replayed from a graph between HIP events on the handle's "
-                  "stream (sph_time_neighbour_passes); other stages: an event pair around each eager stage",
+        "timing": "an event pair on the handle's stream around each stage of eager steps (each interval "
+                  "includes the stage's dispatch gap; rocprofv3's in-step kernel durations are in profiles/)",
         "touched_cells": tc,
         "touched_texels": tex,
         # every neighbour pass and streaming pass of the step on the same footing

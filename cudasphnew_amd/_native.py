@@ -256,7 +256,6 @@ SIGNATURES = [
     ("sph_stage_times", _st, [_vp, fptr, C.POINTER(C.c_int32)]),
     ("sph_stage_name", C.c_char_p, [C.c_int32]),
     ("sph_time_steps", _st, [_vp, C.c_int32, C.c_int32, fptr]),
-    ("sph_time_neighbour_passes", _st, [_vp, C.c_int32, fptr, fptr]),
     ("sph_set_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_get_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_autotune", _st, [_vp, C.c_int32, C.POINTER(SphTuning)]),

@@ -122,7 +122,6 @@ struct Engine {
 
     ~Engine();
     void drop_graph();
-    sph_status time_passes(int reps, float* density_ms, float* force_ms);
     sph_status upload_scenes();
     sph_status alias_distance();   // release the float distance copy (every scene has a network)
     sph_status own_distance();     // allocate and fill it again from grid4

@@ -595,59 +595,6 @@ sph_status Engine::set_tuning(const sph_tuning& t_in) {
     return SPH_OK;
 }
 
-// Device time of one launch of the current density and force variants: reps
-// launches of each captured in a graph and replayed between two events on
-// the handle's stream (a graph runs its kernels back to back, without the
-// dispatch gap each eager launch adds), on the last step's sorted state.
-// Both passes only read the sorted buffers and rewrite what they wrote in
-// that step with the same values, so the state is unchanged.
-sph_status Engine::time_passes(int reps, float* density_ms, float* force_ms) {
-    if (reps < 1 || !density_ms || !force_ms) return fail(SPH_INVALID_ARG, "reps >= 1 and two outputs required");
-    if (!keys_valid) return fail(SPH_INVALID_ARG, "timing the passes needs the state of one step: advance once first");
-    if (timing) {
-        sph_status st = resolve_timing();
-        if (st != SPH_OK) return st;
-    }
-    StepBuffers b = buffers();
-    if (last_two_scale && ts) b.state_b = twoscale_final_state(*this);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        if (e0) (void)hipEventDestroy(e0);
-        return fail(SPH_DEVICE_ERROR, "event creation failed");
-    }
-    bool ok = true;
-    auto time_it = [&](auto&& run) -> float {
-        hipGraph_t g = nullptr;
-        hipGraphExec_t gx = nullptr;
-        float ms = 0.0f;
-        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            ok = false;
-            return 0.0f;
-        }
-        for (int r = 0; r < reps; ++r) run();
-        if (hipStreamEndCapture(stream, &g) != hipSuccess || hipGraphInstantiate(&gx, g, nullptr, nullptr, 0) != hipSuccess ||
-            hipGraphLaunch(gx, stream) != hipSuccess ||   // warm-up
-            hipEventRecord(e0, stream) != hipSuccess || hipGraphLaunch(gx, stream) != hipSuccess ||
-            hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            (void)hipGetLastError();
-            ok = false;
-        } else {
-            (void)hipEventElapsedTime(&ms, e0, e1);
-        }
-        if (gx) (void)hipGraphExecDestroy(gx);
-        if (g) (void)hipGraphDestroy(g);
-        return ms / (float)reps;
-    };
-    *density_ms = time_it([&] { launch_density(stream, b, tuning.density_lanes, tuning.block); });
-    *force_ms = time_it([&] {
-        launch_force(stream, b, tuning.hit_masks ? tuning.hit_list : tuning.force_lanes, tuning.block);
-    });
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (!ok) return fail(SPH_DEVICE_ERROR, "timing the passes: graph capture or replay failed");
-    return check_launch("time passes");
-}
-
 // Times the variants of the density and force kernels on the current sorted
 // state (the inputs of the last step): every lanes-per-particle variant of
 // each, and the hit-mask pair with every hit-list capacity.  Keeps the
@@ -1157,10 +1104,6 @@ sph_status sph_use_graph(sph_handle* h, int32_t enable) {
 
 sph_status sph_time_steps(sph_handle* h, int32_t n, int32_t two_scale, float* ms) {
     return h ? time_engine(h->e, n, two_scale != 0, ms) : SPH_INVALID_ARG;
-}
-
-sph_status sph_time_neighbour_passes(sph_handle* h, int32_t reps, float* density_ms, float* force_ms) {
-    return h ? h->e.time_passes(reps, density_ms, force_ms) : SPH_INVALID_ARG;
 }
 
 sph_status sph_attach_neural_boundary(sph_handle* h, sph_mlp* net, const float* val_xyzd, size_t n_val,

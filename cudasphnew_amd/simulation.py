@@ -240,13 +240,6 @@ class ParticleSimulation:
         check(lib.sph_time_steps(self._need(), int(n), 1 if two_scale else 0, C.byref(ms)), self._h)
         return float(ms.value)
 
-    def time_neighbour_passes(self, reps: int = 20) -> tuple:
-        """(density ms, force ms) per launch: launches of the current variants
-        replayed from a graph on the last step's state
-        (sph_time_neighbour_passes)."""
-        d, f = C.c_float(), C.c_float()
-        check(lib.sph_time_neighbour_passes(self._need(), int(reps), C.byref(d), C.byref(f)), self._h)
-        return float(d.value), float(f.value)
 
     # -- learned boundary -------------------------------------------------------
     def attach_neural(self, net, val=None, cfg=None) -> dict:

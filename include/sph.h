@@ -325,16 +325,6 @@ const char* sph_stage_name(int32_t stage);
 sph_status sph_use_graph(sph_handle* h, int32_t enable);
 sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
                           float* ms);
-/* Device time of one launch of the current density and force variants:
- * reps launches of each captured in a hipGraph and replayed between two HIP
- * events on the handle's stream, on the state of the last step (needs one
- * step first; the state is left unchanged: both passes only read the sorted
- * buffers and rewrite their outputs with the same values).  No reference
- * counterpart (measurement): the kernel time without the dispatch gaps that
- * eager launches and per-stage events include. */
-sph_status sph_time_neighbour_passes(sph_handle* h, int32_t reps,
-                                     float* density_ms, float* force_ms);
-
 /* ---- kernel tuning -----------------------------------------------------
  * Variants of the density and force passes; results agree to FP32 summation
  * order.  density_lanes / force_lanes: lanes of a wavefront that share one

@@ -324,24 +324,6 @@ def test_handle_graph_replays_match_eager(dam16k):
             assert np.array_equal(graph.read(f), eager.read(f)), (rep, f)
 
 
-@pytest.mark.parametrize("tuning", [dict(), dict(hit_masks=1, hit_list=40)], ids=["one-lane", "hit-masks"])
-def test_timing_the_neighbour_passes_leaves_the_state(dam16k, tuning):
-    """sph_time_neighbour_passes re-runs the density and force passes on the
-    last step's state: positive times, and the next steps are bit-equal to
-    those of a handle that was not timed."""
-    a, b = _sim(dam16k), _sim(dam16k)
-    for sim in (a, b):
-        sim.set_tuning(**tuning)
-        sim.Advance(2)
-    d_ms, f_ms = a.time_neighbour_passes(5)
-    assert d_ms > 0.0 and f_ms > 0.0
-    for sim in (a, b):
-        sim.Advance(3)
-        sim.sync()
-    for f in (Field.POSITION, Field.VELOCITY, Field.DENSITY, Field.SORTED_IDS):
-        assert np.array_equal(a.read(f), b.read(f)), f
-
-
 def test_set_params_takes_effect_next_step(dam16k):
     sim = _sim(dam16k)
     p = sim.get_params()
