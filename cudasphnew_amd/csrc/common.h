@@ -48,6 +48,20 @@ struct DevScene {
     uint32_t p_begin, p_end;              // particle range of the scene
 };
 
+// Raw buffer view of a float4 array: a gather takes a 32-bit byte offset
+// (one shift, and one offset register shared by the position and velocity
+// gathers of a neighbour) instead of a 64-bit address per load.  Slots stay
+// below 2^28 (the engine's particle and sub-particle limits).  Measured at 1M particles: force
+// pass 95 → 91 us.
+struct F4Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ explicit F4Buf(const float4* p)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, -1, 0x00020000)) {}
+    __device__ __forceinline__ float4 operator[](uint32_t q) const {
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(q * 16u), 0, 0));
+    }
+};
+
 // Boundary source of a cell when a learned SDF is attached: no wall term (far
 // from every surface), the network's per-particle record, or the grid.
 enum : uint8_t { kWallNone = 0, kWallNeural = 1, kWallGrid = 2 };

@@ -104,8 +104,6 @@ struct Engine {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph_ts = nullptr;
-    hipGraphExec_t graph_exec_ts = nullptr;
 
     TwoScale* ts = nullptr;
     bool last_two_scale = false;

@@ -596,20 +596,6 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
                  key_out, status, nr);
 }
 
-// Raw buffer view of a float4 array: a gather takes a 32-bit byte offset
-// (one shift, and one offset register shared by the position and velocity
-// gathers of a neighbour) instead of a 64-bit address per load.  Slots stay
-// below 2^28 (the engine's particle limit).  Measured at 1M particles: force
-// pass 95 → 91 us.
-struct F4Buf {
-    __amdgpu_buffer_rsrc_t r;
-    __device__ __forceinline__ explicit F4Buf(const float4* p)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(p), (short)0, -1, 0x00020000)) {}
-    __device__ __forceinline__ float4 operator[](uint32_t q) const {
-        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(q * 16u), 0, 0));
-    }
-};
-
 // Hit-mask pipeline.  The density pass already evaluates the distance of
 // every candidate of the nine row runs; it records, per particle and row, the
 // row's first slot and which of its candidates are force neighbours, packed

@@ -616,6 +616,7 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
         const bool split = sperm[t] < nsplit8;   // boundary subs: projected density
         float sum = 0.0f;
         const float4 xi = spos_s[t];
+        const F4Buf sp(spos_s);
         const SubRows R = sub_rows(S, xi);
 #pragma unroll 1
         for (int row = 0; row < 9; ++row) {
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
             uint32_t far = 0;
 #pragma unroll 4
             for (int q = b + lane; q < e; q += kTsLanes) {
-                const float4 xj = spos_s[q];
+                const float4 xj = sp[q];
                 const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
                 float d = fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2)));
                 far = __builtin_amdgcn_alignbit(far, __float_as_uint(d), 31);
@@ -691,11 +692,12 @@ __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, con
         const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;
         const float kr0 = S.stiff * S.rho0;
         float fpx = 0, fpy = 0, fpz = 0, fvx = 0, fvy = 0, fvz = 0, cgx = 0, cgy = 0, cgz = 0, cl = 0;
+        const F4Buf sp(spos_s), sv(svel_s);
         auto pair = [&](int q) {
-            float4 xj = spos_s[q];
+            float4 xj = sp[q];
             float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
             float r2 = dx * dx + dy * dy + dz * dz;
-            float4 vj = svel_s[q];
+            float4 vj = sv[q];
             // a distinct sub at distance 0 contributes nothing (every term
             // carries 1/r or 1/rho_j, both taken as 0 there)
             const bool nz = r2 > 0.0f;
@@ -828,7 +830,7 @@ sph_status twoscale_create(Engine& e) {
     uint64_t cap = P.max_sub_particles ? P.max_sub_particles : 8ull * e.N;
     cap = (cap / 8) * 8;
     if (cap < 8) cap = 8;
-    if (cap > 0x7ffffff8ull) cap = 0x7ffffff8ull;
+    if (cap > (1ull << 28) - 8) cap = (1ull << 28) - 8;   // the sub gathers use 32-bit byte offsets
     t->cap = (uint32_t)cap;
     t->nblk = (e.N + kCompactTile - 1) / kCompactTile;
     t->sub_cells = (uint32_t)sc;

@@ -101,7 +101,7 @@ typedef struct sph_params {
      * 1: children at (+-h/4, +-h/4, +-h/4) around the parent. */
     int32_t sub_placement;
     /* Capacity for active sub-particles; 0 means 8 * num_particles, the
-     * reference allocation (cu:2014-2016). */
+     * reference allocation (cu:2014-2016); at most 268,435,448. */
     uint32_t max_sub_particles;
 } sph_params;
 

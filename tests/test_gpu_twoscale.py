@@ -303,9 +303,9 @@ def test_two_scale_then_plain_steps(dam16k):
 
 
 def test_two_scale_graph_replays_match_eager(dam16k):
-    """The captured two-scale step, replayed between reads and syncs and
-    mixed with plain steps (captured separately), gives the eager results
-    bit for bit."""
+    """With graphs enabled (two-scale steps then run eagerly, plain steps
+    replay their captured graph), two-scale and plain steps mixed between
+    reads and syncs give the all-eager results bit for bit."""
     eager = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
     graph = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
     graph.use_graph(True)
@@ -370,10 +370,11 @@ def test_two_scale_capacity_exceeded_is_reported(dam16k):
 
 
 def test_sub_steps_interleaved_with_graph_replays(dam16k):
-    """AdvanceSubParticles between captured AdvanceTwoScale replays and after
-    plain steps (both sub-particle paths depend on the device-side clearing
-    of the previous sub cells and on slot-consistent coarse state): graph and
-    eager runs agree bit for bit, and the sub-particles follow the oracle."""
+    """AdvanceSubParticles between AdvanceTwoScale steps and after plain
+    steps replayed from their graph (both sub-particle paths depend on the
+    device-side clearing of the previous sub cells and on slot-consistent
+    coarse state): runs with and without graphs agree bit for bit, and the
+    sub-particles follow the oracle."""
     sc = dam16k.with_positions(dam16k.positions)
     eager = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
     graph = S.ParticleSimulation(dam16k.with_positions(dam16k.positions)).Init()
