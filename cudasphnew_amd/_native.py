@@ -256,6 +256,7 @@ SIGNATURES = [
     ("sph_stage_times", _st, [_vp, fptr, C.POINTER(C.c_int32)]),
     ("sph_stage_name", C.c_char_p, [C.c_int32]),
     ("sph_time_steps", _st, [_vp, C.c_int32, C.c_int32, fptr]),
+    ("sph_set_autotune_interval", _st, [_vp, C.c_int32]),
     ("sph_set_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_get_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_autotune", _st, [_vp, C.c_int32, C.POINTER(SphTuning)]),
@@ -271,6 +272,7 @@ SIGNATURES = [
     ("sph_batch_set_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_batch_get_tuning", _st, [_vp, C.POINTER(SphTuning)]),
     ("sph_batch_autotune", _st, [_vp, C.c_int32, C.POINTER(SphTuning)]),
+    ("sph_batch_set_autotune_interval", _st, [_vp, C.c_int32]),
     ("sph_batch_num_particles", C.c_uint64, [_vp]),
     ("sph_batch_last_error_message", C.c_char_p, [_vp]),
     ("sph_scene_config", _st, [C.c_int32, C.c_int32, C.c_uint64, C.POINTER(SceneDesc)]),

@@ -102,6 +102,7 @@ struct Engine {
 
     // graph capture of the steady-state step, plain and two-scale
     bool use_graph = false;
+    uint32_t retune_every = 0;   // re-run the autotuner before every n-th step (0: never)
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 

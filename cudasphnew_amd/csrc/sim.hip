@@ -425,6 +425,10 @@ sph_status Engine::step(int n, bool two_scale) {
         drop_graph();   // the step's buffers now include the sub-particle pass's
     }
     for (int i = 0; i < n; ++i) {
+        if (retune_every && keys_valid && steps > 0 && steps % retune_every == 0) {
+            sph_status st = autotune(2, nullptr);
+            if (st != SPH_OK) return st;
+        }
         // steady state: every buffer pointer and launch of the plain step is
         // fixed, so the step is captured once and replayed.  Two-scale steps
         // run eagerly: replayed from a graph they measured slower (0.250
@@ -1119,6 +1123,13 @@ sph_status sph_detach_neural_boundary(sph_handle* h) {
     return st;
 }
 
+sph_status sph_set_autotune_interval(sph_handle* h, int32_t steps) {
+    if (!h) return SPH_INVALID_ARG;
+    if (steps < 0) return h->e.fail(SPH_INVALID_ARG, "interval must be >= 0");
+    h->e.retune_every = (uint32_t)steps;
+    return SPH_OK;
+}
+
 sph_status sph_set_tuning(sph_handle* h, const sph_tuning* t) {
     return h && t ? h->e.set_tuning(*t) : SPH_INVALID_ARG;
 }
@@ -1199,6 +1210,13 @@ sph_status sph_batch_get_tuning(const sph_batch* b, sph_tuning* out) {
 
 sph_status sph_batch_autotune(sph_batch* b, int32_t reps, sph_tuning* out) {
     return b ? b->e.autotune(reps, out) : SPH_INVALID_ARG;
+}
+
+sph_status sph_batch_set_autotune_interval(sph_batch* b, int32_t steps) {
+    if (!b) return SPH_INVALID_ARG;
+    if (steps < 0) return b->e.fail(SPH_INVALID_ARG, "interval must be >= 0");
+    b->e.retune_every = (uint32_t)steps;
+    return SPH_OK;
 }
 
 sph_status sph_batch_attach_neural_boundary(sph_batch* b, int32_t scene, sph_mlp* net, const float* val_xyzd,

@@ -266,6 +266,10 @@ class ParticleSimulation:
         check(lib.sph_autotune(self._need(), int(reps), C.byref(t)), self._h)
         return _tuning_dict(t)
 
+    def set_autotune_interval(self, steps: int) -> None:
+        """Re-run the autotuner before every `steps`-th step (0: off)."""
+        check(lib.sph_set_autotune_interval(self._need(), int(steps)), self._h)
+
 
 def _tuning_dict(t: SphTuning) -> dict:
     return {f: int(getattr(t, f)) for f, _ in SphTuning._fields_}
@@ -349,3 +353,7 @@ class Batch:
         t = SphTuning()
         check(lib.sph_batch_autotune(self._b, int(reps), C.byref(t)), batch=self._b)
         return _tuning_dict(t)
+
+    def set_autotune_interval(self, steps: int) -> None:
+        """Re-run the autotuner before every `steps`-th step (0: off)."""
+        check(lib.sph_batch_set_autotune_interval(self._b, int(steps)), batch=self._b)

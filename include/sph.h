@@ -353,6 +353,16 @@ sph_status sph_get_tuning(const sph_handle* h, sph_tuning* out);
  * kernel reps times), keeps the fastest, re-runs the last step's density and
  * force with it; out may be NULL. */
 sph_status sph_autotune(sph_handle* h, int32_t reps, sph_tuning* out);
+/* Re-runs sph_autotune (reps 2) before every steps-th step of sph_step and
+ * sph_step_two_scale, on the live state; 0 (the default) turns it off.  The
+ * best variants change as a flow develops: a compressing dam break picks the
+ * hit-mask pair near rest density and the one-lane pair once its particles
+ * have two to three times the neighbours.  Each re-tune times every variant
+ * on the live state (18-56 ms at 1M particles, more as the flow compresses),
+ * synchronises the stream and, with graphs on, re-captures the step: use an
+ * interval of thousands of steps.  No reference counterpart (the reference
+ * has one launch configuration). */
+sph_status sph_set_autotune_interval(sph_handle* h, int32_t steps);
 
 /* ---- batched scenes ----------------------------------------------------
  * Many independent scenes advanced together: one launch per stage over all
@@ -379,6 +389,7 @@ sph_status sph_batch_use_graph(sph_batch* b, int32_t enable);
 sph_status sph_batch_set_tuning(sph_batch* b, const sph_tuning* t);
 sph_status sph_batch_get_tuning(const sph_batch* b, sph_tuning* out);
 sph_status sph_batch_autotune(sph_batch* b, int32_t reps, sph_tuning* out);
+sph_status sph_batch_set_autotune_interval(sph_batch* b, int32_t steps);
 uint64_t sph_batch_num_particles(const sph_batch* b);
 const char* sph_batch_last_error_message(const sph_batch* b);
 

@@ -324,6 +324,26 @@ def test_handle_graph_replays_match_eager(dam16k):
             assert np.array_equal(graph.read(f), eager.read(f)), (rep, f)
 
 
+def test_autotune_interval(dam16k):
+    """sph_set_autotune_interval re-runs the autotuner before every n-th
+    step: the run keeps the step count and stays with a run on fixed
+    variants to FP32 summation order (density 1e-5, positions 1e-6 after 12
+    steps); a negative interval is rejected."""
+    a, b = _sim(dam16k), _sim(dam16k)
+    with pytest.raises(S.SphError):
+        a.set_autotune_interval(-1)
+    a.set_autotune_interval(4)
+    for sim in (a, b):
+        sim.Advance(12)
+        sim.sync()
+    assert a.GetNumTimesSteps() == b.GetNumTimesSteps() == 12
+    rho_a, rho_b = a.read(Field.DENSITY), b.read(Field.DENSITY)
+    assert np.max(np.abs(rho_a - rho_b) / rho_b) <= 1e-5
+    assert np.max(np.abs(a.positions() - b.positions())) <= 1e-6
+    t = a.get_tuning()
+    assert t["block"] in (128, 256)
+
+
 def test_set_params_takes_effect_next_step(dam16k):
     sim = _sim(dam16k)
     p = sim.get_params()
