@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/sph.h"
 
 namespace sph {
@@ -61,6 +63,36 @@ struct F4Buf {
         return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(q * 16u), 0, 0));
     }
 };
+
+// Row words of the hit-mask pair, written by the density pass for each
+// particle and each of its nine neighbour rows (a run of consecutive slots),
+// read by the force pass (and the two-scale marking pass).  Buffer layout:
+// [9][n] 64-bit words, then [3][9][n] 32-bit chunk masks.
+//   word = first slot << 32 | mask of candidates 0..31 | (chunks - 1) << 60
+// where bit t of a mask is set when candidate t of the run is within h (own
+// slot excluded), and a row of c candidates has ceil(c / 32) chunks: chunk
+// k >= 1 (candidates 32k..32k+31) sits at chunk_masks(masks, n)[chunk_at(n,
+// k, row, slot)].  Near rest density a row holds at most 32 candidates (24
+// on a lattice) and only the word is written; a compressed flow's rows reach
+// 40-90.  A row of more than kRowMaxCands candidates gets kRowLong and is
+// scanned directly.  First slots are < 2^28, so bits 62..63 of a word are
+// clear and no word equals kRowLong.
+constexpr int kRowMaxChunks = 4;
+constexpr int kRowMaxCands = 32 * kRowMaxChunks;
+constexpr uint64_t kRowLong = ~0ull;
+// allocation of the buffer in 64-bit units
+constexpr size_t row_words_u64(size_t n) { return 9 * n + ((kRowMaxChunks - 1) * 9 * n + 1) / 2; }
+
+__device__ __forceinline__ uint32_t row_first(uint64_t w) { return (uint32_t)(w >> 32) & 0x0FFFFFFFu; }
+__device__ __forceinline__ uint32_t row_last_chunk(uint64_t w) { return (uint32_t)(w >> 60) & 3u; }
+template <class T>
+__device__ __forceinline__ auto chunk_masks(T* masks, uint32_t n) {
+    using U = typename std::conditional<std::is_const<T>::value, const uint32_t, uint32_t>::type;
+    return reinterpret_cast<U*>(masks + 9 * (size_t)n);
+}
+__device__ __forceinline__ size_t chunk_at(uint32_t n, uint32_t k, int row, uint32_t s) {
+    return ((size_t)(k - 1) * 9 + (size_t)row) * n + s;
+}
 
 // Boundary source of a cell when a learned SDF is attached: no wall term (far
 // from every surface), the network's per-particle record, or the grid.

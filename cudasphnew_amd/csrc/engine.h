@@ -112,7 +112,7 @@ struct Engine {
 
     // kernel variants of the density / force passes
     sph_tuning tuning = {1, 1, 0, 48, 256};
-    uint64_t* masks = nullptr;   // [9][N] per-row force-neighbour masks
+    uint64_t* masks = nullptr;   // row words [9][N] + chunk masks [3][9][N] (common.h)
 
     // render interop buffers, allocated on the first map
     float4* render_pos = nullptr;   // [N] by original id

@@ -37,8 +37,9 @@ struct StepBuffers {
     uint32_t* slot_of_oid = nullptr;
     int2* range = nullptr;       // cell -> [start, end) in sorted order
     uint32_t* status = nullptr;
-    // per-row force-neighbour masks [9][n] written by the density pass and
-    // read by the force pass; null = neighbour loops without masks
+    // row words of the force neighbours (format in common.h), written by the
+    // density pass and read by the force pass; null = neighbour loops
+    // without masks
     uint64_t* masks = nullptr;
     // two-scale split flags, set by the hit-mask density pass (state == null: none)
     SplitRefs split = {nullptr, nullptr, nullptr, nullptr};

@@ -599,15 +599,17 @@ __global__ __launch_bounds__(kBlock) void k_force_coop(
 // Hit-mask pipeline.  The density pass already evaluates the distance of
 // every candidate of the nine row runs; it records, per particle and row, the
 // row's first slot and which of its candidates are force neighbours, packed
-// in one 64-bit word (first slot << 32 | 32-bit mask; masks[row * n + slot],
-// coalesced) when the row holds at most 32 candidates (24 at rest density),
-// and kLongRow otherwise.  The force pass then needs neither the distance
-// tests nor the row runs (no cell-table reads, no neighbour-range
-// divisions): it expands the masks into a per-thread hit list in LDS and
-// runs the pair terms over the list, so a wavefront's trip count is the
-// largest neighbour count of its lanes rather than the sum over rows of the
-// largest per-row count.  Long rows (strong compression) are handled by the
-// force pass directly.
+// in one 64-bit word per row (masks[row * n + slot], coalesced; the format is
+// in common.h): near rest density (at most 32 candidates per row, 24 on a
+// lattice) the word is all there is; the longer rows of a compressed flow
+// add 32-bit masks for their further chunks of 32 candidates, and rows of
+// more than 128 candidates are marked kRowLong.  The force pass then needs
+// neither the distance tests nor the row runs (no cell-table reads, no
+// neighbour-range divisions): it expands the masks into a per-thread hit
+// list in LDS and runs the pair terms over the list, so a wavefront's trip
+// count is the largest neighbour count of its lanes rather than the sum over
+// rows of the largest per-row count.  kRowLong rows are scanned by the force
+// pass directly.
 //
 // Mask bits: r^2 <= h^2, without the particle's own slot.  A distinct
 // particle at distance 0 keeps its bit; the force pass gives it no
@@ -637,13 +639,14 @@ __device__ __forceinline__ void density_candidate(float xjx, float xjy, float xj
     sum += d * d * d;
 }
 
-constexpr uint64_t kLongRow = ~0ull;   // row of more than 32 candidates (first slots are < 2^31)
-
-// The candidates [0, cnt) of one row run; returns the force-neighbour mask
-// of the first 32 (candidate t at bit t; meaningful when cnt <= 32).
+// The candidates [0, cnt) of one row run: adds their poly6 terms to sum and
+// returns the force-neighbour mask of candidates 0..31 (bit t = candidate t
+// within h; self = the particle's own position in the run, or any value >=
+// cnt).  Each further chunk k of 32 candidates is handed to put(k, mask).
 // at(t) -> (x, y, z) of candidate t.
-template <class At>
-__device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi, float h2, float& sum) {
+template <class At, class Put>
+__device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi, float h2, uint32_t self, float& sum,
+                                                const Put& put) {
     uint32_t far = 0;
     const int c32 = min(cnt, 32);
 #pragma unroll 4
@@ -651,13 +654,22 @@ __device__ __forceinline__ uint32_t density_row(const At& at, int cnt, float4 xi
         const float3 p = at(t);
         density_candidate(p.x, p.y, p.z, xi, h2, sum, far);
     }
-    for (int t = 32; t < cnt; ++t) {
-        const float3 p = at(t);
-        float dx = p.x - xi.x, dy = p.y - xi.y, dz = p.z - xi.z;
-        float d = fmaxf(fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, h2))), 0.0f);
-        sum += d * d * d;
+    uint32_t m = near_bits(far, c32);
+    if (self < (uint32_t)c32) m &= ~(1u << self);
+    // rows of a compressed flow
+    for (int c0 = 32; c0 < cnt; c0 += 32) {
+        const int cc = min(cnt - c0, 32);
+        uint32_t f = 0;
+        for (int t = 0; t < cc; ++t) {
+            const float3 p = at(c0 + t);
+            density_candidate(p.x, p.y, p.z, xi, h2, sum, f);
+        }
+        uint32_t mc = near_bits(f, cc);
+        const uint32_t sc = self - (uint32_t)c0;
+        if (sc < (uint32_t)cc) mc &= ~(1u << sc);
+        put((uint32_t)c0 >> 5, mc);
     }
-    return near_bits(far, c32);
+    return m;
 }
 
 // Window of candidate positions per wavefront, in LDS (x, y, z planes).  The
@@ -728,23 +740,30 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         float sum = 0.0f;
+        uint32_t* const cm = chunk_masks(masks, n);
     #pragma unroll
         for (int row = 0; row < 9; ++row) {
             const int b = rb[row], cnt = re[row] - rb[row];
+            // the particle's own slot is not a force neighbour
+            const uint32_t self = s - (uint32_t)b;
+            const bool keep = live && cnt <= kRowMaxCands;
+            auto put = [&](uint32_t k, uint32_t mk) {
+                if (keep) cm[chunk_at(n, k, row, s)] = mk;
+            };
             uint32_t m;
             if (inlds[row]) {
                 const float* w = win + (b + wofs[row]);
                 m = density_row([&](int t) { return make_float3(w[t], w[kWin + t], w[2 * kWin + t]); }, cnt, xi, h2,
-                                sum);
+                                self, sum, put);
             } else {
                 const F4Buf pb(pos);
                 m = density_row([&](int t) { const float4 p = pb[b + t]; return make_float3(p.x, p.y, p.z); }, cnt,
-                                xi, h2, sum);
+                                xi, h2, self, sum, put);
             }
-            // the particle's own slot is not a force neighbour
-            const uint32_t self = s - (uint32_t)b;
-            if (self < (uint32_t)min(cnt, 32)) m &= ~(1u << self);
-            if (live) masks[(size_t)row * n + s] = cnt <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kLongRow;
+            const uint64_t w = (uint64_t)(uint32_t)b << 32 | m;
+            if (live)
+                masks[(size_t)row * n + s] =
+                    cnt <= 32 ? w : (keep ? w | (uint64_t)((cnt - 1) >> 5) << 60 : kRowLong);
         }
         if (live) {
             float rho = sum * (S.poly6 * S.mass) + wall_density(S, grid4, xi, nr, s);
@@ -778,18 +797,26 @@ __global__ __launch_bounds__(B) void k_density_masks(const float4* __restrict__ 
     }
 }
 
-// The hit list holds 16-bit entries (row << 6 | position in the row's run)
-// and the nine run starts per thread: (CAP + 1) * 512 + 9216 bytes of LDS
-// per block, so that five blocks fit the LDS of a CU at CAP = 40 (four at
-// 48, three at 64), and the occupancy hint asks the register allocation for
-// as many wavefronts per SIMD.  Entries past the capacity go to a spare list
-// row and are evaluated afterwards from the masks, so that the expansion
-// loop carries no pair arithmetic; the pair loop keeps the gathers of the
-// next three neighbours in flight while it evaluates the current one.
-// Measured at 1M particles before the packed row words and the scalar scene
-// loads: CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4) 132 us, 32 at 6
-// wavefronts spills and takes 139 us (156 us with two gathers in flight
-// instead of three).
+// The hit list holds 16-bit entries (row << 7 | position in the row's run)
+// and the nine run starts per thread: CAP * 512 + 9216 bytes of LDS per
+// block, so that five blocks fit the LDS of a CU at CAP = 40 (four at 48,
+// three at 64), and the occupancy hint asks the register allocation for as
+// many wavefronts per SIMD.  The list is filled in rounds: the unexpanded
+// hits stay in registers (the current chunk's mask of each row, the chunk
+// index), a round expands up to CAP of them and runs the pair loop over
+// them, and a lane whose round filled the list goes again -- so a compressed
+// flow's longer lists keep the expansion loop free of pair arithmetic and
+// the gathers pipelined.  Measured at 1M particles, hit list 48, on the
+// dam's positions at steps 1000 and 2000 (rows of up to 53 and 88
+// candidates, 12% and 82% of the particles with more than 48 neighbours):
+// 286 -> 121 us and 760 -> 239 us against a single list with the overflow
+// evaluated one gather at a time and rows past 32 candidates scanned; 87 ->
+// 90 us near rest density.  The pair loop keeps the gathers of the next
+// three neighbours in flight while it evaluates the current one.  Measured
+// at 1M particles before the packed row words and the scalar scene loads:
+// CAP 40 (5 wavefronts, 85 VGPRs) 127 us, 48 (4) 132 us, 32 at 6 wavefronts
+// spills and takes 139 us (156 us with two gathers in flight instead of
+// three).
 template <int CAP, int B, bool ONE>
 __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5 : (CAP <= 48 ? 4 : 3), 8))) void k_force_masks(
     const float4* __restrict__ pos, const float4* __restrict__ vel, const int2* __restrict__ range, uint32_t n,
@@ -801,16 +828,32 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5
     NeuralRefs nr) {
     // one column per thread: no barrier needed, consecutive threads hit
     // consecutive banks
-    __shared__ uint16_t s_list[CAP + 1][B];
+    __shared__ uint16_t s_list[CAP][B];
     __shared__ int s_base[9][B];
     WaveSlot ws;
     if (!wave_slot<B, ONE>(n, tiles, ntiles, ws) || !ws.live) return;
     const uint32_t s = ws.s;
     {
         const DevScene& S = scenes[ws.scene];
-        uint64_t mk[9];
+        const int tid = threadIdx.x;
+        // unexpanded hits: bits[row] = the rest of the current chunk's mask;
+        // chunk / last: current and last chunk of each row, 2 bits per row
+        uint32_t bits[9];
+        uint32_t chunk = 0, last = 0, long_rows = 0;
+        {
+            uint64_t mk[9];
     #pragma unroll
-        for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
+            for (int row = 0; row < 9; ++row) mk[row] = masks[(size_t)row * n + s];
+    #pragma unroll
+            for (int row = 0; row < 9; ++row) {
+                const bool lr = mk[row] == kRowLong;
+                long_rows |= (uint32_t)lr << row;
+                bits[row] = lr ? 0u : (uint32_t)mk[row];
+                last |= lr ? 0u : row_last_chunk(mk[row]) << (2 * row);
+                s_base[row][tid] = (int)row_first(mk[row]);
+            }
+        }
+        const uint32_t* const cm = chunk_masks(masks, n);
         const float4 xi = pos[s];
         const float4 vi = vel[s];
         const float rhoi = vi.w;
@@ -818,76 +861,78 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(CAP <= 40 ? 5
         const float Ai = S.stiff * (rhoi - S.rho0) / rhoi;   // p_i / rho_i
         const float kr0 = S.stiff * S.rho0;
         PairSums a = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        int cnt = 0;
-        uint32_t long_rows = 0;
-        const int tid = threadIdx.x;
-    #pragma unroll
-        for (int row = 0; row < 9; ++row) {
-            if (mk[row] == kLongRow) {
-                long_rows |= 1u << row;
-                continue;
-            }
-            s_base[row][tid] = (int)(mk[row] >> 32);
-            uint32_t lo = (uint32_t)mk[row];
-            while (lo) {
-                const int t = __builtin_ctz(lo);
-                lo &= lo - 1;
-                s_list[min(cnt, CAP)][tid] = (uint16_t)(row << 6 | t);
-                ++cnt;
-            }
-        }
-        const int m = min(cnt, CAP);
         const F4Buf pb(pos), vb(vel);
-        if (m > 0) {
-            // the gathers of entries k+1..k+3 are issued before the pair terms
-            // of entry k (short lists repeat their last entry, which is loaded
-            // but not evaluated)
-            auto slot = [&](int k) {
-                const int ent = s_list[min(k, m - 1)][tid];
-                return s_base[ent >> 6][tid] + (ent & 63);
-            };
-            int q = slot(0);
-            float4 x0 = pb[q], v0 = vb[q];
-            q = slot(1);
-            float4 x1 = pb[q], v1 = vb[q];
-            q = slot(2);
-            float4 x2 = pb[q], v2 = vb[q];
-            for (int k = 3; k < m; ++k) {
-                q = slot(k);
-                const float4 xn = pb[q], vn = vb[q];
-                pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
-                x0 = x1; v0 = v1;
-                x1 = x2; v1 = v2;
-                x2 = xn; v2 = vn;
+        int m;
+        do {
+            m = 0;
+    #pragma unroll
+            for (int row = 0; row < 9; ++row) {
+                for (;;) {
+                    const uint32_t c = chunk >> (2 * row) & 3u;
+                    const int off = row << 7 | (int)(c << 5);
+                    uint32_t lo = bits[row];
+                    // a counted loop, unrolled
+                    const int take = min((int)__popc(lo), CAP - m);
+    #pragma unroll 8
+                    for (int i = 0; i < take; ++i) {
+                        const int t = __builtin_ctz(lo);
+                        lo &= lo - 1;
+                        s_list[m + i][tid] = (uint16_t)(off | t);
+                    }
+                    m += take;
+                    bits[row] = lo;
+                    if (lo || c == (last >> (2 * row) & 3u)) break;
+                    // next chunk of a long row
+                    chunk += 1u << (2 * row);
+                    bits[row] = cm[chunk_at(n, c + 1, row, s)];
+                }
             }
-            pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
-            if (m > 1) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
-            if (m > 2) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
-        }
-        if (cnt > CAP || long_rows) {
-            // rare: hit lists longer than the capacity, rows longer than 32
+            if (m > 0) {
+                // three entries in flight: each register set is reloaded with
+                // the entry three places on right after its pair terms, so the
+                // loop (unrolled by three by hand) moves no registers and waits
+                // only for the oldest gathers.  Entries past the end repeat the
+                // last one and are loaded but not evaluated.
+                auto slot = [&](int k) {
+                    const int ent = s_list[min(k, m - 1)][tid];
+                    return s_base[ent >> 7][tid] + (ent & 127);
+                };
+                int q = slot(0);
+                float4 x0 = pb[q], v0 = vb[q];
+                q = slot(1);
+                float4 x1 = pb[q], v1 = vb[q];
+                q = slot(2);
+                float4 x2 = pb[q], v2 = vb[q];
+                int k = 0;
+                for (; k + 3 < m; k += 3) {
+                    pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+                    q = slot(k + 3);
+                    x0 = pb[q]; v0 = vb[q];
+                    pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
+                    q = slot(k + 4);
+                    x1 = pb[q]; v1 = vb[q];
+                    pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
+                    q = slot(k + 5);
+                    x2 = pb[q]; v2 = vb[q];
+                }
+                pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x0, v0);
+                if (k + 1 < m) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x1, v1);
+                if (k + 2 < m) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, x2, v2);
+            }
+        } while (m == CAP);   // a full list: hits may remain
+        if (long_rows) {
+            // rows of more than kRowMaxCands candidates: scanned here
             const Rows R = neighbour_rows(S, xi);
-            int k = 0;
     #pragma unroll 1
             for (int row = 0; row < 9; ++row) {
+                if (!(long_rows >> row & 1u)) continue;
                 int b, e;
-                if (long_rows >> row & 1u) {
-                    row_slots(range, S, R, row / 3, row % 3, xi, b, e);
-                    for (int q = b; q < e; ++q) {
-                        float4 xj = pos[q];
-                        float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                        float r2 = dx * dx + dy * dy + dz * dz;
-                        if (r2 <= h2 && r2 > 0.0f) pair_terms<false>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
-                    }
-                    continue;
-                }
-                b = s_base[row][tid];
-                uint32_t mm = (uint32_t)masks[(size_t)row * n + s];   // reloaded: keeps mk out of the pair loop
-                while (mm) {
-                    const int t = __builtin_ctz(mm);
-                    mm &= mm - 1;
-                    if (k >= CAP) pair_terms<true>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, pos[b + t], vel[b + t]);
-                    ++k;
+                row_slots(range, S, R, row / 3, row % 3, xi, b, e);
+                for (int q = b; q < e; ++q) {
+                    float4 xj = pos[q];
+                    float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    if (r2 <= h2 && r2 > 0.0f) pair_terms<false>(a, xi, vi, rhoi, Ai, h, h2, S.stiff, kr0, xj, vel[q]);
                 }
             }
         }

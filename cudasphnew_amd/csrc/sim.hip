@@ -588,9 +588,9 @@ sph_status Engine::set_tuning(const sph_tuning& t_in) {
     if (t.hit_masks != 0 && t.hit_masks != 1) return fail(SPH_INVALID_ARG, "hit_masks must be 0 or 1");
     if (!list_ok(t.hit_list)) return fail(SPH_INVALID_ARG, "hit_list must be 32, 40, 48 or 64");
     if (t.hit_masks && !masks) {
-        sph_status st = alloc(&masks, (size_t)9 * N);
+        sph_status st = alloc(&masks, row_words_u64(N));
         if (st != SPH_OK) return st;
-        bytes += sizeof(uint64_t) * (size_t)9 * N;
+        bytes += sizeof(uint64_t) * row_words_u64(N);
     }
     if (t.density_lanes != tuning.density_lanes || t.force_lanes != tuning.force_lanes ||
         t.hit_masks != tuning.hit_masks || t.hit_list != tuning.hit_list || t.block != tuning.block)
@@ -613,9 +613,9 @@ sph_status Engine::autotune(int reps, sph_tuning* out) {
         if (st != SPH_OK) return st;
     }
     if (!masks) {
-        sph_status st = alloc(&masks, (size_t)9 * N);
+        sph_status st = alloc(&masks, row_words_u64(N));
         if (st != SPH_OK) return st;
-        bytes += sizeof(uint64_t) * (size_t)9 * N;
+        bytes += sizeof(uint64_t) * row_words_u64(N);
     }
     StepBuffers plain = buffers();
     plain.masks = nullptr;

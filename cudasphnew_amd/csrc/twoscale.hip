@@ -179,12 +179,11 @@ __global__ __launch_bounds__(kB) void k_ts_mark(const float4* __restrict__ pos, 
     }
 }
 
-// The same push driven by the coarse density pass's packed row words (first
-// slot << 32 | mask of the row's candidates within h, own slot excluded;
-// ~0 for a row of more than 32 candidates), when the step runs the hit-mask
-// pair: one lane per split particle and row, no cell-table reads or distance
-// tests for all but long rows.
-constexpr uint64_t kCoarseLongRow = ~0ull;
+// The same push driven by the coarse density pass's row words (first slot,
+// masks of the row's candidates within h in chunks of 32, own slot
+// excluded; kRowLong for a row of more than kRowMaxCands candidates), when
+// the step runs the hit-mask pair: one lane per split particle and row, no
+// cell-table reads or distance tests for all but kRowLong rows.
 
 __global__ __launch_bounds__(kB) void k_ts_mark_rows(const float4* __restrict__ pos, const uint8_t* __restrict__ state,
                                                      const int2* __restrict__ range, const DevScene* __restrict__ scenes,
@@ -199,14 +198,17 @@ __global__ __launch_bounds__(kB) void k_ts_mark_rows(const float4* __restrict__ 
         const int row = (int)(g % 9);
         if (row == 0) state_fin[s] = (uint8_t)(state[s] | 2);   // within h of itself
         const uint64_t w = masks[(size_t)row * n + s];
-        if (w != kCoarseLongRow) {
-            const uint32_t b = (uint32_t)(w >> 32);
-            uint32_t m = (uint32_t)w;
-            while (m) {
-                const uint32_t q = b + (uint32_t)(__ffs(m) - 1);
-                m &= m - 1;
-                const uint8_t sq = state[q];
-                if (!(sq & 1)) state_fin[q] = (uint8_t)(sq | 2);
+        if (w != kRowLong) {
+            const uint32_t last = row_last_chunk(w);
+            for (uint32_t c = 0; c <= last; ++c) {
+                const uint32_t b = row_first(w) + 32u * c;
+                uint32_t m = c == 0 ? (uint32_t)w : chunk_masks(masks, n)[chunk_at(n, c, row, s)];
+                while (m) {
+                    const uint32_t q = b + (uint32_t)(__ffs(m) - 1);
+                    m &= m - 1;
+                    const uint8_t sq = state[q];
+                    if (!(sq & 1)) state_fin[q] = (uint8_t)(sq | 2);
+                }
             }
             continue;
         }

@@ -330,10 +330,11 @@ sph_status sph_time_steps(sph_handle* h, int32_t n_steps, int32_t two_scale,
  * order.  density_lanes / force_lanes: lanes of a wavefront that share one
  * particle in the neighbour loops (1 = one thread per particle; 2, 4, 8, 16 =
  * cooperative variants).  hit_masks = 1: the density pass records the force
- * neighbours of each particle as per-row bit masks (72 bytes per particle of
- * extra device memory) and the force pass runs the pair terms from them
- * through a per-thread hit list of hit_list entries (32, 40, 48 or 64) in
- * LDS;
+ * neighbours of each particle as per-row bit masks (180 bytes per particle
+ * of extra device memory: 72 for rows of up to 32 candidates, the rest for
+ * the longer rows of a compressed flow) and the force pass runs the pair
+ * terms from them through a per-thread hit list of hit_list entries (32, 40,
+ * 48 or 64) in LDS, filled as often as a particle's neighbours need;
  * the lanes fields are then unused.  No reference counterpart: the reference
  * fixes one thread per particle and a block size of 256
  * (particle_simulation.cu:1385-1556, 2943-2957); these generalise its launch

@@ -447,6 +447,40 @@ def test_kernel_variants_dense_cluster(dam16k, tuning):
     assert (np.abs(a - a1) / scale).max() <= 1e-4
 
 
+@pytest.mark.parametrize("squeeze", [1.2, 1.4, 1.8], ids=lambda c: f"x{c}")
+@pytest.mark.parametrize("tuning", [dict(hit_masks=1, hit_list=32), dict(hit_masks=1, hit_list=48),
+                                    dict(hit_masks=1, hit_list=64, block=128)], ids=_tid)
+def test_compressed_block_hit_masks(dam16k, squeeze, tuning):
+    """A block compressed to squeeze^3 times rest density, like the lower
+    layers of a collapsing column: rows of 33-128 candidates (row masks in
+    chunks of 32: about 41 candidates at x1.2, 66 at x1.4), rows of more
+    than 128 candidates at x1.8 (scanned directly), and hit lists of 58 to
+    195 neighbours, several times the list capacity (filled in rounds).
+    Density against the FP64 oracle; acceleration against the oracle and
+    against the one-lane variant."""
+    rng = np.random.default_rng(11)
+    sp = 0.005 / squeeze
+    g = np.arange(16) * sp
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pos = (0.27 + pos + 0.05 * sp * rng.standard_normal(pos.shape)).astype(np.float32)
+    sc = dam16k.with_positions(rng.permutation(pos))
+    ref = _sim(sc)
+    ref.Advance(1)
+    ref.sync()
+    sim = _sim(sc)
+    sim.set_tuning(**tuning)
+    sim.Advance(1)
+    sim.sync()
+    orc = O.OracleSim(sc, real=0, threads=THREADS)
+    orc.step(1)
+    assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
+    a, a1 = sim.read(Field.ACCEL), ref.read(Field.ACCEL)
+    _check_accel(a, orc)
+    scale = np.maximum(np.linalg.norm(a1, axis=1), G)[:, None]
+    assert (np.abs(a - a1) / scale).max() <= 1e-4
+    assert np.max(np.abs(sim.positions() - orc.read(Field.POSITION))) <= 1e-6
+
+
 def test_hit_masks_over_many_steps(dam16k):
     """The hit-mask pipeline stays in parity over 200 steps."""
     sim = _sim(dam16k)
