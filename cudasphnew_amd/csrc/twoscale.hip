@@ -69,7 +69,7 @@ struct TwoScale {
     CellSort csort;
     bool use_cell_sort = false;
     uint32_t* sperm_raw = nullptr;
-    uint64_t* smasks = nullptr;   // [9][cap] packed row words of the split subs (sub density -> sub force)
+    uint64_t* smasks = nullptr;   // row words of the split subs (format in common.h; sub density -> sub force)
     size_t bytes = 0;
     bool sub_cells_valid = false;
 };
@@ -593,9 +593,9 @@ __device__ __forceinline__ void sub_row_slots(const int2* __restrict__ srange, c
     if (e < b) { b = 0; e = 0; }
 }
 
-// Row word of the sub passes: first slot << 32 | force-neighbour mask of the
-// row's first 32 candidates, or kSubLongRow for a row of more candidates.
-constexpr uint64_t kSubLongRow = ~0ull;
+// Row words of the sub passes: the coarse pair's format (common.h) over the
+// sub grid -- first slot, the force-neighbour masks of up to four chunks of
+// 32 candidates, kRowLong for a row of more than kRowMaxCands.
 
 // compute_sub_particle_density_pressure (cu:259-313): split subs only, no
 // wall term.  Besides the density, each split sub's group records per row the
@@ -636,24 +636,31 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
                 d = fmaxf(d, 0.0f);
                 sum += d * d * d;
             }
-            // a row of at most 32 candidates gives each lane at most 8: bit k
-            // of the lane's near bits goes to candidate position 4k + lane
-            uint32_t m = 0;
-            if (e - b <= 32) {
-                const int c = e - b > lane ? (e - b - lane + kTsLanes - 1) / kTsLanes : 0;
-                uint32_t x = c > 0 ? __builtin_bitreverse32(~far) >> (32 - c) : 0u;
+            // a row of at most 128 candidates gives each lane at most 32: bit
+            // k of the lane's near bits goes to candidate position 4k + lane,
+            // chunk k / 8 of the row (the group's lanes share the row, so
+            // they run the chunk loop and its shuffles together)
+            const int cnt = e - b;
+            const int c = cnt > lane ? (cnt - lane + kTsLanes - 1) / kTsLanes : 0;
+            const uint32_t near = c > 0 && cnt <= kRowMaxCands ? __builtin_bitreverse32(~far) >> (32 - c) : 0u;
+            const uint32_t self = t - (uint32_t)b;   // the sub's own slot is no neighbour
+            const int nch = cnt <= kRowMaxCands ? max((cnt + 31) >> 5, 1) : 1;
+            uint32_t m0 = 0;
+            for (int ch = 0; ch < nch; ++ch) {
+                uint32_t x = (near >> (8 * ch)) & 0xFFu;
                 x = (x | (x << 12)) & 0x000F000Fu;
                 x = (x | (x << 6)) & 0x03030303u;
                 x = (x | (x << 3)) & 0x11111111u;
-                m = x << lane;
+                uint32_t m = x << lane;
+                m |= __shfl_xor(m, 1, 64);
+                m |= __shfl_xor(m, 2, 64);
+                if ((self >> 5) == (uint32_t)ch) m &= ~(1u << (self & 31u));
+                if (ch == 0) m0 = m;
+                else if (split && lane == 0) chunk_masks(smasks, cap)[chunk_at(cap, (uint32_t)ch, row, t)] = m;
             }
-            // every lane of the group takes part in the shuffles
-            m |= __shfl_xor(m, 1, 64);
-            m |= __shfl_xor(m, 2, 64);
-            const uint32_t self = t - (uint32_t)b;   // the sub's own slot is no neighbour
-            if (self < 32u) m &= ~(1u << self);
             if (split && lane == 0)
-                smasks[(size_t)row * cap + t] = e - b <= 32 ? ((uint64_t)(uint32_t)b << 32 | m) : kSubLongRow;
+                smasks[(size_t)row * cap + t] =
+                    cnt <= kRowMaxCands ? ((uint64_t)(uint32_t)b << 32 | m0 | (uint64_t)(nch - 1) << 60) : kRowLong;
         }
         sum = group_sum(sum);   // every lane of the group takes part
         if (split && lane == 0) reinterpret_cast<float*>(svel_s)[4 * (size_t)t + 3] = sum * (S.poly6_sub * S.mass_sub);
@@ -666,7 +673,10 @@ __global__ __launch_bounds__(kB) void k_ts_density(const TSCounts* __restrict__ 
 // sub-particle.  The pair terms run over the set bits of the density pass's
 // row words, in candidate order (rows in z-major, y-minor order, slots
 // ascending), so a wavefront executes them as often as its lane with the most
-// neighbours in the row; a row of more than 32 candidates is tested here.
+// neighbours in the row; a row of more than kRowMaxCands candidates is tested
+// here.  (Before the rows kept chunks past 32 candidates, such rows were
+// tested here too: on the two-scale workload after ~1,000 steps, as the
+// flow compresses, this pass took 120 us against 44 us now.)
 __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, const uint32_t* __restrict__ sperm,
                                                  const float4* __restrict__ spos_s, const float4* __restrict__ svel_s,
                                                  const int2* __restrict__ srange, const DevScene* __restrict__ scenes,
@@ -725,13 +735,16 @@ __global__ __launch_bounds__(kB) void k_ts_force(TSCounts* __restrict__ cnt, con
 #pragma unroll 1
         for (int row = 0; row < 9; ++row) {
             const uint64_t w = smasks[(size_t)row * cap + t];
-            if (w != kSubLongRow) {
-                const int b = (int)(w >> 32);
-                uint32_t m = (uint32_t)w;
-                while (m) {
-                    const int u = __ffs(m) - 1;
-                    m &= m - 1;
-                    pair(b + u);
+            if (w != kRowLong) {
+                const uint32_t last = row_last_chunk(w);
+                for (uint32_t ch = 0; ch <= last; ++ch) {
+                    const int b = (int)(row_first(w) + 32u * ch);
+                    uint32_t m = ch == 0 ? (uint32_t)w : chunk_masks(smasks, cap)[chunk_at(cap, ch, row, t)];
+                    while (m) {
+                        const int u = __ffs(m) - 1;
+                        m &= m - 1;
+                        pair(b + u);
+                    }
                 }
             } else {
                 int b, e;
@@ -849,7 +862,7 @@ sph_status twoscale_create(Engine& e) {
               ts_alloc(*t, &t->spos_s, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_s, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->spos_out, t->cap) == hipSuccess && ts_alloc(*t, &t->svel_out, t->cap) == hipSuccess &&
               ts_alloc(*t, &t->sacc, t->cap) == hipSuccess && ts_alloc(*t, &t->srange, t->sub_cells) == hipSuccess &&
-              ts_alloc(*t, &t->smasks, 9 * (size_t)t->cap) == hipSuccess;
+              ts_alloc(*t, &t->smasks, row_words_u64(t->cap)) == hipSuccess;
     t->use_cell_sort = (uint64_t)t->sub_cells <= (uint64_t)kCellSortRatio * t->cap;
     if (ok && t->use_cell_sort)
         ok = ts_alloc(*t, &t->sperm_raw, t->cap) == hipSuccess && t->csort.init(t->sub_cells, t->cap) == hipSuccess;

@@ -226,6 +226,30 @@ def test_two_scale_with_hit_masks(dam16k):
     assert np.max(np.abs(sim.read(Field.DENSITY) / orc.read(Field.DENSITY) - 1)) <= 1e-5
 
 
+@pytest.mark.parametrize("squeeze", [1.4, 1.8], ids=lambda c: f"x{c}")
+def test_two_scale_compressed_sub_rows(dam16k, squeeze):
+    """A block compressed to squeeze^3 times rest density inside the split
+    region: every parent splits, and the sub rows hold about 66 (x1.4) and
+    140 (x1.8) candidates -- row masks in chunks of 32, and rows past 128
+    candidates scanned by the sub force pass.  Sub grid bit-exact; sub
+    density and acceleration against the FP64 oracle on the device's own
+    sub-particle state."""
+    rng = np.random.default_rng(5)
+    sp = float(np.cbrt(dam16k.params.particle_mass / dam16k.params.rest_density)) / squeeze
+    g = np.arange(16) * sp
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + np.array([0.27, 0.2, 0.2])
+    pos = (pos + 0.05 * sp * rng.standard_normal(pos.shape)).astype(np.float32)
+    sc = dam16k.with_positions(rng.permutation(pos))
+    sim = S.ParticleSimulation(sc).Init()
+    sim.set_tuning(hit_masks=1, hit_list=48)
+    sim.AdvanceTwoScale(1)
+    sim.sync()
+    c = sim.counts()
+    assert c.num_split == pos.shape[0]
+    _check_sub_grid_exact(sim, sc.params)
+    _check_sub_physics_same_inputs(sim, sc)
+
+
 def test_two_scale_state_history_over_steps(dam16k):
     sc = dam16k.with_positions(dam16k.positions)
     sim = S.ParticleSimulation(sc).Init()
