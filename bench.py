@@ -232,6 +232,8 @@ def kernel_name(stage: str, tuning: dict, one_scene: bool = True) -> str:
             return f"k_density_masks<{b},{one}>"
         return f"k_density<{b},{one}>" if tuning["density_lanes"] == 1 else \
             f"k_density_coop<{tuning['density_lanes']}>"
+    if stage == "reorder":
+        return f"k_reorder<{'false' if one_scene else 'true'}>"
     return f"k_{stage}"
 
 

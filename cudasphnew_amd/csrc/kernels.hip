@@ -114,20 +114,49 @@ __global__ __launch_bounds__(kBlock) void k_cells(const uint32_t* __restrict__ s
 // arrival (counting sort) or storage (radix sort) order; the reference sorts
 // (hash, original id), so inside each cell run the slot is the rank of the
 // original id (runs are short: a cell holds ~8 particles at rest density).
+// SHUFFLE: a run that lies inside the lane's wavefront is ranked with
+// shuffles of the lanes' ids, in a loop as long as the wavefront's longest
+// such run, instead of reading the run's ids (a run crossing the
+// wavefront's edge still reads them).  Measured: 82.8 -> 76.2 us for the
+// 64-scene batch (cells of up to 17 particles), but 15.7 -> 17.6 us for the
+// 1M scene (8 per cell, the reads hit the cache), so batches use it.
+template <bool SHUFFLE>
 __global__ __launch_bounds__(kBlock) void k_reorder(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ perm, const int2* __restrict__ range,
     const uint32_t* __restrict__ oid_a, const float4* __restrict__ pos_a, const float4* __restrict__ vel_a,
     const uint8_t* __restrict__ state_a, uint32_t n, float4* __restrict__ pos_b, float4* __restrict__ vel_b,
     uint32_t* __restrict__ oid_b, uint8_t* __restrict__ state_b, uint32_t* __restrict__ slot_of_oid) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    uint32_t k = skey[s];
-    int2 r = range[k];
-    uint32_t src = perm[s];
-    uint32_t me = oid_a[src];
-    uint32_t rank = 0;
-    for (int q = r.x; q < r.y; ++q) rank += (oid_a[perm[q]] < me) ? 1u : 0u;
-    uint32_t dst = (uint32_t)r.x + rank;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k, src, me, rank = 0;
+    int2 r;
+    if (SHUFFLE) {
+        const uint32_t w0 = s & ~63u;   // slot of the wavefront's first lane
+        if (w0 >= n) return;            // the whole wavefront lies past the end
+        const bool valid = s < n;
+        k = valid ? skey[s] : 0u;
+        r = valid ? range[k] : make_int2(0, 0);
+        src = valid ? perm[s] : 0u;
+        me = valid ? oid_a[src] : 0u;
+        const bool inwave = valid && (uint32_t)r.x >= w0 && (uint32_t)r.y <= w0 + 64u;
+        const int len = inwave ? r.y - r.x : 0;
+        const int maxlen = __builtin_amdgcn_readfirstlane(__ockl_wfred_max_i32(len));
+        const int first = inwave ? r.x - (int)w0 : 0;   // lane of the run's first slot
+        for (int j = 0; j < maxlen; ++j) {   // every lane of the wavefront takes part
+            const uint32_t o = __shfl(me, first + min(j, max(len - 1, 0)), 64);
+            rank += (j < len && o < me) ? 1u : 0u;
+        }
+        if (!valid) return;
+        if (!inwave)
+            for (int q = r.x; q < r.y; ++q) rank += (oid_a[perm[q]] < me) ? 1u : 0u;
+    } else {
+        if (s >= n) return;
+        k = skey[s];
+        r = range[k];
+        src = perm[s];
+        me = oid_a[src];
+        for (int q = r.x; q < r.y; ++q) rank += (oid_a[perm[q]] < me) ? 1u : 0u;
+    }
+    const uint32_t dst = (uint32_t)r.x + rank;
     pos_b[dst] = pos_a[src];
     vel_b[dst] = vel_a[src];
     oid_b[dst] = me;
@@ -1036,9 +1065,14 @@ void launch_cells(hipStream_t st, const StepBuffers& b) {
 }
 
 void launch_reorder(hipStream_t st, const StepBuffers& b) {
-    hipLaunchKernelGGL(k_reorder, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.skey, b.perm, b.range,
-                       b.oid_a, b.pos_a, b.vel_a, b.state_a, b.n, b.pos_b, b.vel_b, b.oid_b, b.state_b,
-                       b.slot_of_oid);
+    if (b.nscenes > 1)
+        hipLaunchKernelGGL(k_reorder<true>, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.skey, b.perm,
+                           b.range, b.oid_a, b.pos_a, b.vel_a, b.state_a, b.n, b.pos_b, b.vel_b, b.oid_b, b.state_b,
+                           b.slot_of_oid);
+    else
+        hipLaunchKernelGGL(k_reorder<false>, dim3(blocks_for(b.n, kBlock)), dim3(kBlock), 0, st, b.skey, b.perm,
+                           b.range, b.oid_a, b.pos_a, b.vel_a, b.state_a, b.n, b.pos_b, b.vel_b, b.oid_b, b.state_b,
+                           b.slot_of_oid);
 }
 
 // Workgroups of the hit-mask kernels: one thread per slot, or (batch) one
